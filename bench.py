@@ -1,0 +1,191 @@
+"""Benchmark: explicit internal-force step on the 128^3 two-block HEX8 cube.
+
+One step = one central-difference step of the whole mesh on the device:
+fused element kernel (F -> Neo-Hookean stress -> B^T.sigma) per block, CSR
+node gather fused with the velocity / displacement update, and the top-face
+reaction sum.  Inputs are resident in HBM before timing starts.
+
+    python bench.py [--gpus 1] [--steps 500] [--warmup 20] [--precision mixed|fp64]
+                    [--workload cube128_2blk] [--cpu-steps 8] [--no-cpu]
+
+Prints one JSON line.  `value` is element-steps/s of the timed K steps.
+`roofline` covers the dominant (element) kernel: algorithmic bytes per launch
+(connectivity + F and sigma written once per Gauss point + X and u read once
+per node) over its average launch duration from HIP events on the model's
+stream.  `cpu_baseline` times the FP64 CPU oracle (OpenMP, deterministic
+gather) on a bounded sample of the same workload.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
+
+
+def algorithmic_bytes(prob, s_bytes):
+    """Entry-point byte model.  Per block launch of the element kernel:
+    E_b*(32 + 120*s) + N_b*48 (N_b = distinct nodes of the block); per step:
+    E*(32 + 120*s) + N*152."""
+    per_launch = []
+    for (e0, e1) in prob.block_ranges():
+        nb = len(np.unique(prob.conn[e0:e1]))
+        per_launch.append((e1 - e0) * (32 + 120 * s_bytes) + nb * 48)
+    step = prob.num_elem * (32 + 120 * s_bytes) + prob.num_nodes * 152
+    return per_launch, step
+
+
+def load_traffic(workload, precision):
+    """HBM bytes per element-kernel launch measured with rocprofv3 --pmc
+    (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel calibration), as
+    committed under profiles/; None when not measured for this workload."""
+    path = os.path.join(ROOT, "profiles", f"element_{precision}_{workload}_traffic.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        return json.load(f).get("bytes_per_launch")
+
+
+def cpu_baseline(prob, steps):
+    """FP64 CPU oracle, OpenMP over elements + deterministic CSR node gather."""
+    from oracle import oracle as O  # the checker: only this leg of bench.py touches it
+    from tests.helpers import oracle_model_from_problem
+
+    threads = min(16, os.cpu_count() or 1)
+    om = oracle_model_from_problem(prob)
+    dt = prob.dt_factor * om.critical_dt()
+    run = O.ExplicitRun(om, prob.num_steps * dt, prob.num_steps, 0, threads=threads)
+    run.advance(1)  # warm-up (page-in, OpenMP pool)
+    t0 = time.perf_counter()
+    run.advance(steps)
+    el = time.perf_counter() - t0
+    return {
+        "value": prob.num_elem * steps / el,
+        "unit": "element-steps/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": f"{steps} FP64 steps of {prob.name} ({prob.num_elem} elements) after 1 warm-up step, "
+                  f"{threads} OpenMP threads, {el:.1f} s",
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=500)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--precision", default="mixed", choices=["mixed", "fp64"])
+    ap.add_argument("--workload", default="cube128_2blk")
+    ap.add_argument("--cpu-steps", type=int, default=8)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py: this build measures one GPU only; multi-GPU runs (domain decomposition with a "
+              "shared-node force exchange) are not part of it. Use --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    import nimblesm_amd as N
+
+    if not torch.cuda.is_available():
+        print("bench.py: no HIP device visible", file=sys.stderr)
+        sys.exit(1)
+    torch.cuda.init()
+
+    prob = N.workload(args.workload).build()
+    s_bytes = 4 if args.precision == "mixed" else 8
+    total = args.warmup + args.steps
+    model = prob.make_model(args.precision)
+
+    # warm-up run, then restart from rest for the timed run of K steps
+    integ = prob.make_integrator(model, num_steps=max(total, 1))
+    integ.Advance(args.warmup)
+    model.Synchronize()
+    zeros = np.zeros((prob.num_nodes, 3))
+    for label in ("displacement", "velocity", "acceleration", "internal_force"):
+        model.SetVectorNodeData(label, zeros)
+    integ = prob.make_integrator(model, num_steps=args.steps)
+
+    model.EnableKernelTiming(True)
+    torch.cuda.synchronize()
+    model.Synchronize()
+    t0 = time.perf_counter()
+    integ.Advance(args.steps)
+    model.Synchronize()
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+
+    el_ms, el_n = model.KernelTime("element")
+    nd_ms, nd_n = model.KernelTime("node")
+    rx_ms, rx_n = model.KernelTime("reaction")
+    model.EnableKernelTiming(False)
+
+    per_launch, step_bytes = algorithmic_bytes(prob, s_bytes)
+    launches_per_step = len(per_launch)
+    # mean algorithmic bytes per launch over the launches timed (blocks alternate)
+    bytes_per_launch = float(np.mean(per_launch))
+    achieved = bytes_per_launch / (el_ms * 1e-3) / 1e9 if el_ms > 0 else 0.0
+    traffic = load_traffic(args.workload, args.precision)
+    ms_per_step = elapsed * 1e3 / args.steps
+    value = prob.num_elem * args.steps / elapsed
+    R = model.GetReactionHistory(0, args.steps)
+
+    out = {
+        "metric": "element-steps/s + % HBM roofline",
+        "value": value,
+        "unit": "element-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32+f64" if args.precision == "mixed" else "f64",
+        "data": "synthetic (seeded jittered cube, SplitMix64)",
+        "config": {
+            "workload": args.workload,
+            "elements": prob.num_elem,
+            "nodes": prob.num_nodes,
+            "blocks": len(prob.materials),
+            "precision": args.precision,
+            "steps": args.steps,
+        },
+        "roofline": {
+            "bound": "hbm",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": traffic,
+        },
+        "kernels": {
+            "element_ms_avg": el_ms, "element_launches": el_n, "element_launches_per_step": launches_per_step,
+            "element_alg_bytes_per_launch": bytes_per_launch,
+            "node_ms_avg": nd_ms, "node_launches": nd_n,
+            "reaction_ms_avg": rx_ms, "reaction_launches": rx_n,
+        },
+        "step_roofline": {
+            "alg_bytes_per_step": step_bytes,
+            "achieved_GBs": step_bytes / (ms_per_step * 1e-3) / 1e9,
+            "frac": step_bytes / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS,
+        },
+        "reaction_final": float(R[-1]),
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(prob, args.cpu_steps)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

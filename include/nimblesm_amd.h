@@ -1,0 +1,193 @@
+/*
+ * nimblesm_amd.h -- C ABI of the MI355X (gfx950) explicit internal-force
+ * library.  Plain C types only: sizes are int64_t, arrays are raw pointers,
+ * a stream is an opaque void* (a hipStream_t, NULL = the null stream).
+ *
+ * Every entry point returns an nsm_status; on failure nsm_last_error()
+ * returns a message for the calling thread.  No entry point throws.
+ *
+ * Conventions (identical to NimbleSM):
+ *   node vectors       [N][3] double
+ *   connectivity       [E][8] int32, 0-based, HEX8 node order of
+ *                      src/nimble_element.cc:100-121
+ *   gathered element   [E][8][3] double (the Kokkos "gathered" views,
+ *   arrays             src/nimble_kokkos_model_data.cc:1636-1658)
+ *   deformation grad.  [.][8][9] order XX,YY,ZZ,XY,YZ,ZX,YX,ZY,XZ
+ *   Cauchy stress      [.][8][6] order XX,YY,ZZ,XY,YZ,ZX
+ *                      (src/nimble_utils.h:91-109)
+ *   internal force     carries the reference's minus sign
+ *                      (src/nimble_element.h:614-616)
+ */
+#ifndef NIMBLESM_AMD_H
+#define NIMBLESM_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum nsm_status {
+  NSM_OK               = 0,
+  NSM_ERR_ARG          = 1, /* invalid argument / size                        */
+  NSM_ERR_HIP          = 2, /* HIP runtime error                              */
+  NSM_ERR_NEG_JACOBIAN = 3, /* det <= 0 in Invert3x3 (src/nimble_utils.h:1243) */
+  NSM_ERR_STATE        = 4, /* call out of order (e.g. no lumped mass yet)    */
+  NSM_ERR_NOMEM        = 5  /* device allocation failed                       */
+} nsm_status;
+
+typedef enum nsm_precision {
+  NSM_FP64  = 0, /* plain FP64, reference operation order                     */
+  NSM_MIXED = 1  /* FP32 F / stress / B^T.sigma, FP64 node state and sums     */
+} nsm_precision;
+
+typedef enum nsm_field {
+  NSM_FIELD_REFERENCE_COORDINATE = 0, /* [N][3] */
+  NSM_FIELD_DISPLACEMENT         = 1, /* [N][3] */
+  NSM_FIELD_VELOCITY             = 2, /* [N][3] */
+  NSM_FIELD_ACCELERATION         = 3, /* [N][3] */
+  NSM_FIELD_INTERNAL_FORCE       = 4, /* [N][3] */
+  NSM_FIELD_LUMPED_MASS          = 5, /* [N]    */
+  NSM_FIELD_DEFORMATION_GRADIENT = 6, /* [E][8][9], STEP_NP1 */
+  NSM_FIELD_STRESS               = 7  /* [E][8][6], STEP_NP1 */
+} nsm_field;
+
+typedef enum nsm_bc_kind {
+  NSM_BC_INITIAL_VELOCITY        = 0,
+  NSM_BC_PRESCRIBED_VELOCITY     = 1,
+  NSM_BC_PRESCRIBED_DISPLACEMENT = 2
+} nsm_bc_kind;
+
+/* kernels whose launch durations nsm_model_kernel_time() reports */
+typedef enum nsm_kernel {
+  NSM_KERNEL_ELEMENT = 0, /* fused F -> stress -> B^T.sigma element kernel      */
+  NSM_KERNEL_NODE    = 1, /* CSR gather + central-difference node kernel         */
+  NSM_KERNEL_REACTION = 2 /* reaction-force sum                                  */
+} nsm_kernel;
+
+const char* nsm_status_string(int status);
+const char* nsm_last_error(void);
+/* Library version and the device it was built for ("gfx950"). */
+const char* nsm_version(void);
+
+/* ---------------------------------------------------------------------------
+ * Element seam: HexElement on gathered per-element arrays (device pointers).
+ * ------------------------------------------------------------------------- */
+
+/* Replaces HexElement::ComputeDeformationGradients (Kokkos overload,
+ * src/nimble_element.h:430-502 via :353-361): F at the 8 Gauss points with
+ * current coordinates x = X + u. */
+int nsm_hex_compute_deformation_gradients(int64_t num_elem, const double* X_gathered, const double* u_gathered,
+                                          double* F, void* stream);
+
+/* Replaces HexElement::ComputeNodalForces (src/nimble_element.h:539-625 via
+ * :644-653): f = -sum_q (dN/dx . sigma_q) det(J_q) w_q, per element. */
+int nsm_hex_compute_nodal_forces(int64_t num_elem, const double* X_gathered, const double* u_gathered,
+                                 const double* sigma, double* f, void* stream);
+
+/* Replaces HexElement::ComputeLumpedMass (src/nimble_element.cc:173-193,
+ * src/nimble_element.h:266-309): row sums of the consistent mass, [E][8]. */
+int nsm_hex_compute_lumped_mass(int64_t num_elem, double density, const double* X_gathered, double* m,
+                                void* stream);
+
+/* Replaces HexElement::ComputeCharacteristicLength
+ * (src/nimble_element.cc:221-260) on current coordinates, [E]. */
+int nsm_hex_compute_characteristic_length(int64_t num_elem, const double* x_gathered, double* out, void* stream);
+
+/* ---------------------------------------------------------------------------
+ * Material seam.
+ * ------------------------------------------------------------------------- */
+
+/* Replaces NeohookeanMaterial::GetStress (array form
+ * src/nimble_material.h:240-253, per point src/nimble_material.cc:252-310;
+ * device per-point form src/nimble_material.h:269-286) for num_points
+ * points: sigma_np1 from F_np1 through Polar_Decomp. */
+int nsm_neohookean_get_stress(int64_t num_points, double bulk_modulus, double shear_modulus, const double* F_np1,
+                              double* sigma_np1, void* stream);
+
+/* ---------------------------------------------------------------------------
+ * Model-data seam: replaces nimble::ModelDataBase / nimble_kokkos::ModelData
+ * (src/nimble_model_data_base.h:87-345, src/nimble_kokkos_model_data.cc) and
+ * the central-difference loop of ExplicitTimeIntegrator::Integrate
+ * (src/integrators/explicit_time_integrator.cc:177-278).  The model owns its
+ * device arrays and its HIP stream; host pointers below are host memory.
+ * ------------------------------------------------------------------------- */
+typedef struct nsm_model nsm_model;
+
+/* Blocks are given in the order they are stored (the reference iterates
+ * blocks in ascending block id, src/nimble_model_data.cc:636-659); conn holds
+ * the connectivity of all blocks concatenated in that order. */
+int nsm_model_create(int64_t num_nodes, const double* reference_coordinates, int num_blocks, const int* block_ids,
+                     const int64_t* block_num_elem, const int* conn, const double* density,
+                     const double* bulk_modulus, const double* shear_modulus, int precision, nsm_model** out);
+int nsm_model_destroy(nsm_model* model);
+
+/* Replaces BoundaryConditionManager's constant-magnitude initial / prescribed
+ * velocity and prescribed displacement conditions
+ * (src/nimble_boundary_condition_manager.h:93-217).  Conditions apply in the
+ * order they are added, as the reference's list does. */
+int nsm_model_add_boundary_condition(nsm_model* model, int kind, int coordinate, double magnitude, int64_t num_nodes,
+                                     const int* node_ids);
+/* ModelDataBase::ApplyInitialConditions / ApplyKinematicConditions
+ * (src/nimble_model_data_base.cc:84-100). */
+int nsm_model_apply_initial_conditions(nsm_model* model);
+int nsm_model_apply_kinematic_conditions(nsm_model* model, double time_current, double time_previous);
+
+/* ModelData::ComputeLumpedMass (src/nimble_kokkos_model_data.cc:1067-1152):
+ * lumped mass by a deterministic node gather in (block, element) order, and
+ * the critical time step min_e charlen_e / sqrt(K/rho)
+ * (src/nimble_block_base.cc:52-84). */
+int nsm_model_compute_lumped_mass(nsm_model* model);
+int nsm_model_get_critical_time_step(nsm_model* model, double* dt);
+
+/* ModelData::ComputeInternalForce (src/nimble_kokkos_model_data.cc:1154-1286):
+ * f_int, F_np1 and sigma_np1 from the model's current displacement. */
+int nsm_model_compute_internal_force(nsm_model* model, double time_previous, double time_current,
+                                     int is_output_step);
+
+/* ModelData::UpdateStates (src/nimble_kokkos_model_data.cc:963-1021).  The
+ * stateless Neo-Hookean model never reads STEP_N, so this is a no-op kept for
+ * interface parity. */
+int nsm_model_update_states(nsm_model* model);
+
+/* Reaction-force history: after every step, -sum over node_ids of
+ * f_int[node][coordinate] (FP64).  The reference has no such output; this is
+ * the quantity of interest of the synthetic benchmarks. */
+int nsm_model_set_reaction_nodes(nsm_model* model, int64_t num_nodes, const int* node_ids, int coordinate);
+int nsm_model_get_reaction_history(nsm_model* model, int64_t first_step, int64_t count, double* out);
+
+/* Steps [step_begin, step_end) of ExplicitTimeIntegrator::Integrate with
+ * t += dt_user, delta = t - t_prev; *time_current is read and updated.  All
+ * work stays on the device.  Requires nsm_model_compute_lumped_mass. */
+int nsm_model_explicit_steps(nsm_model* model, int64_t step_begin, int64_t step_end, int64_t num_load_steps,
+                             int64_t output_frequency, double dt_user, double* time_current);
+
+/* Host copies of fields (ModelDataBase::GetVectorNodeData /
+ * GetScalarNodeData, src/nimble_model_data_base.h:145-173).  Integration
+ * point fields are returned as double whatever the precision. */
+int nsm_model_get_field(nsm_model* model, int field, double* out);
+int nsm_model_set_field(nsm_model* model, int field, const double* in);
+/* Device address of a field (node fields double; ipt fields double in FP64
+ * mode and float in mixed mode) and its element size in bytes. */
+int nsm_model_field_device_pointer(nsm_model* model, int field, void** ptr, int* elem_bytes);
+
+int nsm_model_synchronize(nsm_model* model);
+/* Launch-duration statistics gathered by HIP events around every launch of
+ * `kernel` while timing is enabled: average milliseconds and launch count. */
+int nsm_model_enable_kernel_timing(nsm_model* model, int enable);
+int nsm_model_kernel_time(nsm_model* model, int kernel, double* avg_ms, int64_t* launches);
+/* Sizes of the model: nodes, elements, CSR entries, bytes of device memory. */
+int nsm_model_info(nsm_model* model, int64_t* num_nodes, int64_t* num_elem, int64_t* csr_entries,
+                   int64_t* device_bytes);
+
+/* ---------------------------------------------------------------------------
+ * Synthetic input: n^3 HEX8 unit cube, node (i,j,k) -> i+(n+1)(j+(n+1)k),
+ * interior nodes jittered by U(-f,f)/n per axis from SplitMix64 keyed by
+ * (seed, 3*node+axis).  Host arrays X [(n+1)^3][3], conn [n^3][8].
+ * ------------------------------------------------------------------------- */
+int nsm_jittered_cube(int n, uint64_t seed, double jitter_fraction, double* X, int* conn);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

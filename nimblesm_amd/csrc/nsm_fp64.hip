@@ -1,0 +1,508 @@
+// nsm_fp64.hip -- FP64 parity path: fused HEX8 element kernel (F -> Neo-Hookean
+// stress -> B^T.sigma into per-element force slots), the element / material
+// seam kernels and the lumped-mass / critical-time-step setup kernels.
+//
+// Compiled with -ffp-contract=off: every FP64 expression keeps NimbleSM's
+// evaluation order so results match the reference's FP64 run bit for bit
+// (cbrt aside).  Work mapping: one 64-wide wavefront holds 8 elements, lane
+// k of an element serves as node k (gather, force output) and as Gauss point
+// k (F, stress); LDS carries the per-element node data and per-point
+// inverse Jacobians between the two roles.
+#include "nsm_fp64_math.h"
+#include "nsm_kernels.h"
+
+namespace nsm {
+namespace {
+
+// ---------------------------------------------------------------------------
+// HEX8 tables, evaluated at compile time with IEEE double arithmetic in the
+// order of src/nimble_element.cc:55-171 (Gauss constant written as in :58).
+// ---------------------------------------------------------------------------
+struct HexTables {
+  double N[64];    // [ipt][node]
+  double dN[192];  // [ipt][node][dir]
+};
+
+constexpr HexTables make_hex_tables()
+{
+  HexTables    t{};
+  const double val     = 0.577350269189626;
+  const int    sg[8][3] = {{-1, -1, -1}, {1, -1, -1}, {1, 1, -1}, {-1, 1, -1},
+                           {-1, -1, 1},  {1, -1, 1},  {1, 1, 1},  {-1, 1, 1}};
+  const double c       = 1.0 / 8.0;
+  for (int q = 0; q < 8; q++) {
+    const double r = sg[q][0] > 0 ? val : -val;
+    const double s = sg[q][1] > 0 ? val : -val;
+    const double u = sg[q][2] > 0 ? val : -val;
+    t.N[8 * q + 0] = c * (1.0 - r) * (1.0 - s) * (1.0 - u);
+    t.N[8 * q + 1] = c * (1.0 + r) * (1.0 - s) * (1.0 - u);
+    t.N[8 * q + 2] = c * (1.0 + r) * (1.0 + s) * (1.0 - u);
+    t.N[8 * q + 3] = c * (1.0 - r) * (1.0 + s) * (1.0 - u);
+    t.N[8 * q + 4] = c * (1.0 - r) * (1.0 - s) * (1.0 + u);
+    t.N[8 * q + 5] = c * (1.0 + r) * (1.0 - s) * (1.0 + u);
+    t.N[8 * q + 6] = c * (1.0 + r) * (1.0 + s) * (1.0 + u);
+    t.N[8 * q + 7] = c * (1.0 - r) * (1.0 + s) * (1.0 + u);
+    double* d = &t.dN[24 * q];
+    d[0]  = -c * (1.0 - s) * (1.0 - u);
+    d[1]  = -c * (1.0 - r) * (1.0 - u);
+    d[2]  = -c * (1.0 - r) * (1.0 - s);
+    d[3]  = c * (1.0 - s) * (1.0 - u);
+    d[4]  = -c * (1.0 + r) * (1.0 - u);
+    d[5]  = -c * (1.0 + r) * (1.0 - s);
+    d[6]  = c * (1.0 + s) * (1.0 - u);
+    d[7]  = c * (1.0 + r) * (1.0 - u);
+    d[8]  = -c * (1.0 + r) * (1.0 + s);
+    d[9]  = -c * (1.0 + s) * (1.0 - u);
+    d[10] = c * (1.0 - r) * (1.0 - u);
+    d[11] = -c * (1.0 - r) * (1.0 + s);
+    d[12] = -c * (1.0 - s) * (1.0 + u);
+    d[13] = -c * (1.0 - r) * (1.0 + u);
+    d[14] = c * (1.0 - r) * (1.0 - s);
+    d[15] = c * (1.0 - s) * (1.0 + u);
+    d[16] = -c * (1.0 + r) * (1.0 + u);
+    d[17] = c * (1.0 + r) * (1.0 - s);
+    d[18] = c * (1.0 + s) * (1.0 + u);
+    d[19] = c * (1.0 + r) * (1.0 + u);
+    d[20] = c * (1.0 + r) * (1.0 + s);
+    d[21] = -c * (1.0 + s) * (1.0 + u);
+    d[22] = c * (1.0 - r) * (1.0 + u);
+    d[23] = c * (1.0 - r) * (1.0 + s);
+  }
+  return t;
+}
+
+__constant__ HexTables c_hex = make_hex_tables();
+
+constexpr int kEPW     = 32;        // elements per 256-thread workgroup
+constexpr int kThreads = 8 * kEPW;  // 8 lanes per element
+
+using namespace fp64;
+
+// a = sum_j x_j (x) dN_j/dxi and b = sum_j X_j (x) dN_j/dxi at ipt q with
+// x = X + u (src/nimble_element.h:442-482).  xs/us are [8][3].
+__device__ __forceinline__ void jacobians(int q, const double (*xs)[3], const double (*us)[3], double a[9],
+                                          double b[9])
+{
+  for (int i = 0; i < 9; i++) a[i] = b[i] = 0.0;
+#pragma unroll
+  for (int j = 0; j < 8; j++) {
+    const double rc1 = xs[j][0], rc2 = xs[j][1], rc3 = xs[j][2];
+    const double cc1 = xs[j][0] + us[j][0];
+    const double cc2 = xs[j][1] + us[j][1];
+    const double cc3 = xs[j][2] + us[j][2];
+    const double s1 = c_hex.dN[24 * q + 3 * j], s2 = c_hex.dN[24 * q + 3 * j + 1], s3 = c_hex.dN[24 * q + 3 * j + 2];
+    a[0] += cc1 * s1;
+    a[1] += cc1 * s2;
+    a[2] += cc1 * s3;
+    a[3] += cc2 * s1;
+    a[4] += cc2 * s2;
+    a[5] += cc2 * s3;
+    a[6] += cc3 * s1;
+    a[7] += cc3 * s2;
+    a[8] += cc3 * s3;
+    b[0] += rc1 * s1;
+    b[1] += rc1 * s2;
+    b[2] += rc1 * s3;
+    b[3] += rc2 * s1;
+    b[4] += rc2 * s2;
+    b[5] += rc2 * s3;
+    b[6] += rc3 * s1;
+    b[7] += rc3 * s2;
+    b[8] += rc3 * s3;
+  }
+}
+
+// F = a b^{-1} in K_F storage (src/nimble_element.h:484-500)
+__device__ __forceinline__ void def_grad(const double a[9], const double bi[9], double F[9])
+{
+  double dg[9];
+#pragma unroll
+  for (int j = 0; j < 3; j++)
+#pragma unroll
+    for (int k = 0; k < 3; k++) dg[3 * j + k] = a[3 * j] * bi[k] + a[3 * j + 1] * bi[3 + k] + a[3 * j + 2] * bi[6 + k];
+  F[F_XX] = dg[0];
+  F[F_XY] = dg[1];
+  F[F_XZ] = dg[2];
+  F[F_YX] = dg[3];
+  F[F_YY] = dg[4];
+  F[F_YZ] = dg[5];
+  F[F_ZX] = dg[6];
+  F[F_ZY] = dg[7];
+  F[F_ZZ] = dg[8];
+}
+
+// Force on node n from all 8 points, in ipt order, from the per-point inverse
+// Jacobian / det (ai[q][0..8], ai[q][9]) and stress sg[q][0..5]
+// (src/nimble_element.h:586-617).
+__device__ __forceinline__ void node_force(int n, const double (*ai)[10], const double (*sg)[6], double f[3])
+{
+  f[0] = f[1] = f[2] = 0.0;
+#pragma unroll
+  for (int q = 0; q < 8; q++) {
+    const double d0 = c_hex.dN[24 * q + 3 * n], d1 = c_hex.dN[24 * q + 3 * n + 1], d2 = c_hex.dN[24 * q + 3 * n + 2];
+    const double* A  = ai[q];
+    const double* s  = sg[q];
+    const double  x1 = d0 * A[0] + d1 * A[3] + d2 * A[6];
+    const double  x2 = d0 * A[1] + d1 * A[4] + d2 * A[7];
+    const double  x3 = d0 * A[2] + d1 * A[5] + d2 * A[8];
+    double        f1 = x1 * s[S_XX] + x2 * s[S_XY] + x3 * s[S_ZX];
+    double        f2 = x1 * s[S_XY] + x2 * s[S_YY] + x3 * s[S_YZ];
+    double        f3 = x1 * s[S_ZX] + x2 * s[S_YZ] + x3 * s[S_ZZ];
+    const double  jw = A[9] * 1.0;
+    f1 *= jw;
+    f2 *= jw;
+    f3 *= jw;
+    f[0] -= f1;
+    f[1] -= f2;
+    f[2] -= f3;
+  }
+}
+
+struct Lds {
+  union {
+    struct {
+      double x[kEPW][8][3];
+      double u[kEPW][8][3];
+    } nodes;
+    struct {
+      double ai[kEPW][8][10];
+      double sg[kEPW][8][6];
+    } pts;
+  };
+};
+
+__global__ __launch_bounds__(kThreads) void k_element_fp64(int64_t e_begin, int64_t e_end,
+                                                           const int* __restrict__ conn,
+                                                           const double* __restrict__ X,
+                                                           const double* __restrict__ u, double K, double G,
+                                                           double* __restrict__ Fip, double* __restrict__ sip,
+                                                           double* __restrict__ slots, int* __restrict__ status)
+{
+  __shared__ Lds lds;
+  const int     el    = threadIdx.x >> 3;
+  const int     k     = threadIdx.x & 7;
+  const int64_t e     = e_begin + (int64_t)blockIdx.x * kEPW + el;
+  const bool    valid = e < e_end;
+
+  if (valid) {
+    const int64_t nd = conn[e * 8 + k];
+    lds.nodes.x[el][k][0] = X[3 * nd];
+    lds.nodes.x[el][k][1] = X[3 * nd + 1];
+    lds.nodes.x[el][k][2] = X[3 * nd + 2];
+    lds.nodes.u[el][k][0] = u[3 * nd];
+    lds.nodes.u[el][k][1] = u[3 * nd + 1];
+    lds.nodes.u[el][k][2] = u[3 * nd + 2];
+  }
+  __syncthreads();
+
+  double ainv[10], sig[6];
+  int    bad = 0;
+  if (valid) {
+    double a[9], b[9], bi[9], F[9];
+    jacobians(k, lds.nodes.x[el], lds.nodes.u[el], a, b);
+    bad |= invert3x3(b, bi) <= 0.0;
+    def_grad(a, bi, F);
+    double* Fo = Fip + (e * 8 + k) * 9;
+#pragma unroll
+    for (int i = 0; i < 9; i++) Fo[i] = F[i];
+    neohookean_stress(K, G, F, sig);
+    double* so = sip + (e * 8 + k) * 6;
+#pragma unroll
+    for (int i = 0; i < 6; i++) so[i] = sig[i];
+    // the force routine rebuilds a from the same x = X + u in the same order
+    ainv[9] = invert3x3(a, ainv);
+    bad |= ainv[9] <= 0.0;
+  }
+  __syncthreads();  // node data no longer needed
+  if (valid) {
+#pragma unroll
+    for (int i = 0; i < 10; i++) lds.pts.ai[el][k][i] = ainv[i];
+#pragma unroll
+    for (int i = 0; i < 6; i++) lds.pts.sg[el][k][i] = sig[i];
+  }
+  __syncthreads();
+  if (valid) {
+    double f[3];
+    node_force(k, lds.pts.ai[el], lds.pts.sg[el], f);
+    double* so = slots + (e * 8 + k) * 3;
+    so[0]      = f[0];
+    so[1]      = f[1];
+    so[2]      = f[2];
+  }
+  if (bad) atomicOr(status, 1);
+}
+
+// ---------------------------------------------------------------------------
+// Seams on gathered arrays
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kThreads) void k_seam_defgrad(int64_t E, const double* __restrict__ Xg,
+                                                           const double* __restrict__ ug, double* __restrict__ F,
+                                                           int* __restrict__ status)
+{
+  __shared__ double xs[kEPW][8][3], us[kEPW][8][3];
+  const int     el = threadIdx.x >> 3, k = threadIdx.x & 7;
+  const int64_t e  = (int64_t)blockIdx.x * kEPW + el;
+  if (e < E)
+    for (int c = 0; c < 3; c++) {
+      xs[el][k][c] = Xg[(e * 8 + k) * 3 + c];
+      us[el][k][c] = ug[(e * 8 + k) * 3 + c];
+    }
+  __syncthreads();
+  if (e < E) {
+    double a[9], b[9], bi[9], Fq[9];
+    jacobians(k, xs[el], us[el], a, b);
+    if (invert3x3(b, bi) <= 0.0) atomicOr(status, 1);
+    def_grad(a, bi, Fq);
+    for (int i = 0; i < 9; i++) F[(e * 8 + k) * 9 + i] = Fq[i];
+  }
+}
+
+__global__ __launch_bounds__(kThreads) void k_seam_forces(int64_t E, const double* __restrict__ Xg,
+                                                          const double* __restrict__ ug,
+                                                          const double* __restrict__ sig, double* __restrict__ f,
+                                                          int* __restrict__ status)
+{
+  __shared__ Lds lds;
+  const int     el = threadIdx.x >> 3, k = threadIdx.x & 7;
+  const int64_t e  = (int64_t)blockIdx.x * kEPW + el;
+  if (e < E)
+    for (int c = 0; c < 3; c++) {
+      lds.nodes.x[el][k][c] = Xg[(e * 8 + k) * 3 + c];
+      lds.nodes.u[el][k][c] = ug[(e * 8 + k) * 3 + c];
+    }
+  __syncthreads();
+  double ainv[10];
+  if (e < E) {
+    double a[9], b[9];
+    jacobians(k, lds.nodes.x[el], lds.nodes.u[el], a, b);
+    ainv[9] = invert3x3(a, ainv);
+    if (ainv[9] <= 0.0) atomicOr(status, 1);
+  }
+  __syncthreads();
+  if (e < E) {
+    for (int i = 0; i < 10; i++) lds.pts.ai[el][k][i] = ainv[i];
+    for (int i = 0; i < 6; i++) lds.pts.sg[el][k][i] = sig[(e * 8 + k) * 6 + i];
+  }
+  __syncthreads();
+  if (e < E) {
+    double fn[3];
+    node_force(k, lds.pts.ai[el], lds.pts.sg[el], fn);
+    for (int c = 0; c < 3; c++) f[(e * 8 + k) * 3 + c] = fn[c];
+  }
+}
+
+__global__ __launch_bounds__(256) void k_seam_stress(int64_t P, double K, double G, const double* __restrict__ F,
+                                                     double* __restrict__ sig)
+{
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= P) return;
+  double Fp[9], s[6];
+  for (int i = 0; i < 9; i++) Fp[i] = F[p * 9 + i];
+  neohookean_stress(K, G, Fp, s);
+  for (int i = 0; i < 6; i++) sig[p * 6 + i] = s[i];
+}
+
+// Lumped mass: lane q forms det J_q from reference coordinates, lane i then
+// sums row i of the consistent mass (src/nimble_element.h:266-309 and
+// src/nimble_element.cc:189-192).
+__device__ __forceinline__ void lumped_row(int i, double rho, const double* detJ, double* out)
+{
+  double lm = 0.0;
+  for (int j = 0; j < 8; j++) {
+    double cm = 0.0;
+    for (int q = 0; q < 8; q++) cm += 1.0 * rho * c_hex.N[q * 8 + i] * c_hex.N[q * 8 + j] * detJ[q];
+    lm += cm;
+  }
+  *out = lm;
+}
+
+__global__ __launch_bounds__(kThreads) void k_seam_lumped_mass(int64_t E, double rho, const double* __restrict__ Xg,
+                                                               double* __restrict__ m, int* __restrict__ status)
+{
+  __shared__ double xs[kEPW][8][3], zs[kEPW][8][3], det[kEPW][8];
+  const int     el = threadIdx.x >> 3, k = threadIdx.x & 7;
+  const int64_t e  = (int64_t)blockIdx.x * kEPW + el;
+  if (e < E)
+    for (int c = 0; c < 3; c++) {
+      xs[el][k][c] = Xg[(e * 8 + k) * 3 + c];
+      zs[el][k][c] = 0.0;
+    }
+  __syncthreads();
+  if (e < E) {
+    double a[9], b[9], bi[9];
+    jacobians(k, xs[el], zs[el], a, b);
+    det[el][k] = invert3x3(b, bi);
+    if (det[el][k] <= 0.0) atomicOr(status, 1);
+  }
+  __syncthreads();
+  if (e < E) lumped_row(k, rho, det[el], &m[e * 8 + k]);
+}
+
+__device__ __forceinline__ double char_length(const double* x)
+{
+  // src/nimble_element.cc:221-260 (x_max starts at 0.0 as in the reference)
+  double x_min, x_max, y_min, y_max, z_min, z_max, min_d2;
+  x_max = y_max = z_max = 0.0;
+  min_d2 = x_min = y_min = z_min = 1.7976931348623157e308;
+  for (int n = 0; n < 8; n++) {
+    const double nx = x[3 * n], ny = x[3 * n + 1], nz = x[3 * n + 2];
+    if (nx < x_min) x_min = nx;
+    if (nx > x_max) x_max = nx;
+    if (ny < y_min) y_min = ny;
+    if (ny > y_max) y_max = ny;
+    if (nz < z_min) z_min = nz;
+    if (nz > z_max) z_max = nz;
+    for (int m = n + 1; m < 8; m++) {
+      const double mx = x[3 * m], my = x[3 * m + 1], mz = x[3 * m + 2];
+      const double d2 = (nx - mx) * (nx - mx) + (ny - my) * (ny - my) + (nz - mz) * (nz - mz);
+      if (d2 < min_d2) min_d2 = d2;
+    }
+  }
+  double cl      = sqrt(min_d2);
+  double min_box = x_max - x_min;
+  if (y_max - y_min < min_box) min_box = y_max - y_min;
+  if (z_max - z_min < min_box) min_box = z_max - z_min;
+  if (min_box < cl) cl = min_box;
+  return cl;
+}
+
+__global__ __launch_bounds__(256) void k_seam_char_length(int64_t E, const double* __restrict__ xg,
+                                                          double* __restrict__ out)
+{
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= E) return;
+  double x[24];
+  for (int i = 0; i < 24; i++) x[i] = xg[e * 24 + i];
+  out[e] = char_length(x);
+}
+
+// ---------------------------------------------------------------------------
+// Model setup
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kThreads) void k_element_mass(int64_t e_begin, int64_t e_end,
+                                                           const int* __restrict__ conn,
+                                                           const double* __restrict__ X, double rho,
+                                                           double* __restrict__ elem_mass, int* __restrict__ status)
+{
+  __shared__ double xs[kEPW][8][3], zs[kEPW][8][3], det[kEPW][8];
+  const int     el = threadIdx.x >> 3, k = threadIdx.x & 7;
+  const int64_t e  = e_begin + (int64_t)blockIdx.x * kEPW + el;
+  if (e < e_end) {
+    const int64_t nd = conn[e * 8 + k];
+    for (int c = 0; c < 3; c++) {
+      xs[el][k][c] = X[3 * nd + c];
+      zs[el][k][c] = 0.0;
+    }
+  }
+  __syncthreads();
+  if (e < e_end) {
+    double a[9], b[9], bi[9];
+    jacobians(k, xs[el], zs[el], a, b);
+    det[el][k] = invert3x3(b, bi);
+    if (det[el][k] <= 0.0) atomicOr(status, 1);
+  }
+  __syncthreads();
+  if (e < e_end) lumped_row(k, rho, det[el], &elem_mass[e * 8 + k]);
+}
+
+// Minimum over the workgroup's elements of charlen / sound_speed on current
+// coordinates X + u (src/nimble_block_base.cc:52-84); wave-level shuffles
+// then LDS across the 4 waves.  min is exact in any order.
+__global__ __launch_bounds__(256) void k_element_dt(int64_t e_begin, int64_t e_end, const int* __restrict__ conn,
+                                                    const double* __restrict__ X, const double* __restrict__ u,
+                                                    double sound_speed, double* __restrict__ wg_min)
+{
+  __shared__ double wmin[4];
+  const int64_t     e  = e_begin + (int64_t)blockIdx.x * 256 + threadIdx.x;
+  double            dt = 1.7976931348623157e308;
+  if (e < e_end) {
+    double x[24];
+    for (int n = 0; n < 8; n++) {
+      const int64_t nd = conn[e * 8 + n];
+      for (int c = 0; c < 3; c++) x[3 * n + c] = X[3 * nd + c] + u[3 * nd + c];
+    }
+    dt = char_length(x) / sound_speed;
+  }
+  for (int off = 32; off > 0; off >>= 1) dt = fmin(dt, __shfl_xor(dt, off, 64));
+  if ((threadIdx.x & 63) == 0) wmin[threadIdx.x >> 6] = dt;
+  __syncthreads();
+  if (threadIdx.x == 0) wg_min[blockIdx.x] = fmin(fmin(wmin[0], wmin[1]), fmin(wmin[2], wmin[3]));
+}
+
+__global__ __launch_bounds__(256) void k_min_reduce(const double* __restrict__ in, int64_t n, double* __restrict__ out)
+{
+  __shared__ double wmin[4];
+  double            m = 1.7976931348623157e308;
+  for (int64_t i = threadIdx.x; i < n; i += 256) m = fmin(m, in[i]);
+  for (int off = 32; off > 0; off >>= 1) m = fmin(m, __shfl_xor(m, off, 64));
+  if ((threadIdx.x & 63) == 0) wmin[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) *out = fmin(fmin(wmin[0], wmin[1]), fmin(wmin[2], wmin[3]));
+}
+
+inline unsigned grid_of(int64_t n, int64_t per) { return (unsigned)((n + per - 1) / per); }
+
+}  // namespace
+
+void launch_element_fp64(int64_t e_begin, int64_t e_end, const int* conn, const double* X, const double* u, double K,
+                         double G, double* Fip, double* sip, double* slots, int* status, hipStream_t s)
+{
+  if (e_end <= e_begin) return;
+  hipLaunchKernelGGL(k_element_fp64, dim3(grid_of(e_end - e_begin, kEPW)), dim3(kThreads), 0, s, e_begin, e_end,
+                     conn, X, u, K, G, Fip, sip, slots, status);
+}
+
+void launch_seam_defgrad(int64_t E, const double* Xg, const double* ug, double* F, int* status, hipStream_t s)
+{
+  if (E <= 0) return;
+  hipLaunchKernelGGL(k_seam_defgrad, dim3(grid_of(E, kEPW)), dim3(kThreads), 0, s, E, Xg, ug, F, status);
+}
+
+void launch_seam_forces(int64_t E, const double* Xg, const double* ug, const double* sig, double* f, int* status,
+                        hipStream_t s)
+{
+  if (E <= 0) return;
+  hipLaunchKernelGGL(k_seam_forces, dim3(grid_of(E, kEPW)), dim3(kThreads), 0, s, E, Xg, ug, sig, f, status);
+}
+
+void launch_seam_stress(int64_t P, double K, double G, const double* F, double* sig, hipStream_t s)
+{
+  if (P <= 0) return;
+  hipLaunchKernelGGL(k_seam_stress, dim3(grid_of(P, 256)), dim3(256), 0, s, P, K, G, F, sig);
+}
+
+void launch_seam_lumped_mass(int64_t E, double rho, const double* Xg, double* m, int* status, hipStream_t s)
+{
+  if (E <= 0) return;
+  hipLaunchKernelGGL(k_seam_lumped_mass, dim3(grid_of(E, kEPW)), dim3(kThreads), 0, s, E, rho, Xg, m, status);
+}
+
+void launch_seam_char_length(int64_t E, const double* xg, double* out, hipStream_t s)
+{
+  if (E <= 0) return;
+  hipLaunchKernelGGL(k_seam_char_length, dim3(grid_of(E, 256)), dim3(256), 0, s, E, xg, out);
+}
+
+void launch_element_mass(int64_t e_begin, int64_t e_end, const int* conn, const double* X, double rho,
+                         double* elem_mass, int* status, hipStream_t s)
+{
+  if (e_end <= e_begin) return;
+  hipLaunchKernelGGL(k_element_mass, dim3(grid_of(e_end - e_begin, kEPW)), dim3(kThreads), 0, s, e_begin, e_end,
+                     conn, X, rho, elem_mass, status);
+}
+
+int launch_element_dt(int64_t e_begin, int64_t e_end, const int* conn, const double* X, const double* u,
+                      double sound_speed, double* wg_min, hipStream_t s)
+{
+  if (e_end <= e_begin) return 0;
+  const unsigned g = grid_of(e_end - e_begin, 256);
+  hipLaunchKernelGGL(k_element_dt, dim3(g), dim3(256), 0, s, e_begin, e_end, conn, X, u, sound_speed, wg_min);
+  return (int)g;
+}
+
+void launch_min_reduce(const double* in, int64_t n, double* out, hipStream_t s)
+{
+  hipLaunchKernelGGL(k_min_reduce, dim3(1), dim3(256), 0, s, in, n, out);
+}
+
+}  // namespace nsm

@@ -1,0 +1,957 @@
+// nsm_model.cpp -- host side of the C ABI: the device-resident ModelData,
+// the explicit central-difference driver and the element / material seams.
+//
+// The model keeps every node and integration-point array in HBM for its whole
+// life; a step is a short sequence of stream-ordered launches (element kernel
+// per block, node kernel, optional reaction sum) with no host round trip.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/nimblesm_amd.h"
+#include "nsm_kernels.h"
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int fail(int status, const char* fmt, ...)
+{
+  char    buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  g_last_error = buf;
+  return status;
+}
+
+#define NSM_HIP(call)                                                                              \
+  do {                                                                                             \
+    hipError_t _e = (call);                                                                        \
+    if (_e != hipSuccess) return fail(NSM_ERR_HIP, "%s: %s", #call, hipGetErrorString(_e));        \
+  } while (0)
+
+#define NSM_LAUNCHED()                                                                             \
+  do {                                                                                             \
+    hipError_t _e = hipGetLastError();                                                             \
+    if (_e != hipSuccess) return fail(NSM_ERR_HIP, "kernel launch: %s", hipGetErrorString(_e));    \
+  } while (0)
+
+// Small device scratch shared by the stateless seam entry points: one status
+// word per device, allocated on first use and kept for the process.
+std::mutex       g_seam_mutex;
+std::map<int, int*> g_seam_status;
+
+int seam_status_buffer(int** out)
+{
+  int dev = 0;
+  NSM_HIP(hipGetDevice(&dev));
+  std::lock_guard<std::mutex> lk(g_seam_mutex);
+  auto                        it = g_seam_status.find(dev);
+  if (it == g_seam_status.end()) {
+    int* p = nullptr;
+    NSM_HIP(hipMalloc(&p, sizeof(int)));
+    it = g_seam_status.emplace(dev, p).first;
+  }
+  *out = it->second;
+  return NSM_OK;
+}
+
+template <typename T>
+int dalloc(T** p, size_t count, size_t* total)
+{
+  *p = nullptr;
+  if (count == 0) return NSM_OK;
+  hipError_t e = hipMalloc((void**)p, count * sizeof(T));
+  if (e != hipSuccess) {
+    *p = nullptr;
+    return fail(NSM_ERR_NOMEM, "hipMalloc(%zu bytes): %s", count * sizeof(T), hipGetErrorString(e));
+  }
+  *total += count * sizeof(T);
+  return NSM_OK;
+}
+
+struct BcRecord {
+  int              kind, coord;
+  double           value;
+  std::vector<int> nodes;
+};
+
+struct EventPair {
+  hipEvent_t start, stop;
+};
+
+}  // namespace
+
+struct nsm_model {
+  int         device    = 0;
+  hipStream_t stream    = nullptr;
+  int         precision = NSM_FP64;
+
+  int64_t              N = 0, E = 0;
+  int                  num_blocks = 0;
+  std::vector<int>     block_ids;
+  std::vector<int64_t> blk_off;
+  std::vector<double>  rho, K, G;
+
+  // device arrays
+  double*   X = nullptr;
+  double*   u = nullptr;
+  double*   v = nullptr;
+  double*   a = nullptr;
+  double*   f = nullptr;
+  double*   m = nullptr;
+  int*      conn = nullptr;
+  uint32_t* node_ptr = nullptr;
+  uint32_t* node_ent = nullptr;
+  void*     Fip = nullptr;    // double or float [E][8][9]
+  void*     sip = nullptr;    // double or float [E][8][6]
+  void*     slots = nullptr;  // double or float [E][8][3]
+  double*   elem_tmp = nullptr;  // [8E] setup scratch (element masses)
+  double*   wg_min = nullptr;
+  int64_t   wg_min_len = 0;
+  int*      status = nullptr;
+  uint8_t*  bc_code = nullptr;
+  nsm::BcEntry* bc_table = nullptr;
+  int*      reaction_nodes = nullptr;
+  int64_t   num_reaction_nodes = 0;
+  int       reaction_coord = 2;
+  double*   reaction_hist = nullptr;
+  int64_t   reaction_cap = 0;
+  size_t    device_bytes = 0;
+
+  std::vector<BcRecord> bcs;
+  bool                  bc_dirty = true;
+  int                   num_bc_entries = 0;
+  bool                  have_mass = false;
+  double                critical_dt = 0.0;
+
+  bool                                timing = false;
+  std::vector<EventPair>              event_pool;
+  size_t                              event_next = 0;
+  std::vector<std::pair<int, size_t>> timed;  // (kernel, event index)
+
+  nsm::NodeArrays node_arrays() const
+  {
+    nsm::NodeArrays na;
+    na.num_nodes = N;
+    na.node_ptr  = node_ptr;
+    na.node_ent  = node_ent;
+    na.mass      = m;
+    na.u         = u;
+    na.v         = v;
+    na.a         = a;
+    na.f         = f;
+    na.bc_code   = bc_code;
+    na.bc_table  = bc_table;
+    return na;
+  }
+};
+
+namespace {
+
+int ensure_device(nsm_model* md)
+{
+  NSM_HIP(hipSetDevice(md->device));
+  return NSM_OK;
+}
+
+int time_begin(nsm_model* md, int kernel, size_t* idx)
+{
+  *idx = (size_t)-1;
+  if (!md->timing) return NSM_OK;
+  if (md->event_next == md->event_pool.size()) {
+    EventPair p;
+    NSM_HIP(hipEventCreate(&p.start));
+    NSM_HIP(hipEventCreate(&p.stop));
+    md->event_pool.push_back(p);
+  }
+  *idx = md->event_next++;
+  NSM_HIP(hipEventRecord(md->event_pool[*idx].start, md->stream));
+  md->timed.emplace_back(kernel, *idx);
+  return NSM_OK;
+}
+
+int time_end(nsm_model* md, size_t idx)
+{
+  if (idx == (size_t)-1) return NSM_OK;
+  NSM_HIP(hipEventRecord(md->event_pool[idx].stop, md->stream));
+  return NSM_OK;
+}
+
+// Resolve the prescribed conditions, in insertion order, to one (kind, value)
+// per node and coordinate: later conditions overwrite earlier ones exactly as
+// the reference's sequential loops do.
+int rebuild_bcs(nsm_model* md)
+{
+  if (!md->bc_dirty) return NSM_OK;
+  std::vector<nsm::BcEntry> per_node;
+  std::vector<int>          touched;
+  std::map<int, size_t>     slot_of;
+  for (const auto& b : md->bcs) {
+    if (b.kind == NSM_BC_INITIAL_VELOCITY) continue;
+    for (int n : b.nodes) {
+      auto it = slot_of.find(n);
+      if (it == slot_of.end()) {
+        nsm::BcEntry z;
+        std::memset(&z, 0, sizeof(z));
+        it = slot_of.emplace(n, per_node.size()).first;
+        per_node.push_back(z);
+        touched.push_back(n);
+      }
+      per_node[it->second].kind[b.coord]  = b.kind;
+      per_node[it->second].value[b.coord] = b.value;
+    }
+  }
+  // distinct combinations -> table
+  std::vector<nsm::BcEntry> table;
+  std::vector<uint8_t>      code((size_t)md->N, 0);
+  for (size_t t = 0; t < touched.size(); t++) {
+    const nsm::BcEntry& e   = per_node[t];
+    int                 idx = -1;
+    for (size_t j = 0; j < table.size(); j++)
+      if (std::memcmp(&table[j], &e, sizeof(e)) == 0) {
+        idx = (int)j;
+        break;
+      }
+    if (idx < 0) {
+      if (table.size() >= 255)
+        return fail(NSM_ERR_ARG, "more than 255 distinct kinematic-condition combinations are not supported");
+      idx = (int)table.size();
+      table.push_back(e);
+    }
+    code[(size_t)touched[t]] = (uint8_t)(idx + 1);
+  }
+  if (md->bc_table) {
+    NSM_HIP(hipFree(md->bc_table));
+    md->bc_table = nullptr;
+  }
+  size_t dummy = 0;
+  int    st    = dalloc(&md->bc_table, std::max<size_t>(table.size(), 1), &dummy);
+  if (st) return st;
+  if (!table.empty())
+    NSM_HIP(hipMemcpy(md->bc_table, table.data(), table.size() * sizeof(nsm::BcEntry), hipMemcpyHostToDevice));
+  NSM_HIP(hipMemcpy(md->bc_code, code.data(), code.size(), hipMemcpyHostToDevice));
+  md->num_bc_entries = (int)table.size();
+  md->bc_dirty       = false;
+  return NSM_OK;
+}
+
+int check_status(nsm_model* md)
+{
+  int h = 0;
+  NSM_HIP(hipMemcpyAsync(&h, md->status, sizeof(int), hipMemcpyDeviceToHost, md->stream));
+  NSM_HIP(hipStreamSynchronize(md->stream));
+  if (h) {
+    NSM_HIP(hipMemsetAsync(md->status, 0, sizeof(int), md->stream));
+    NSM_HIP(hipStreamSynchronize(md->stream));
+    return fail(NSM_ERR_NEG_JACOBIAN, "non-positive Jacobian determinant in an element (Invert3x3)");
+  }
+  return NSM_OK;
+}
+
+int launch_elements(nsm_model* md)
+{
+  for (int b = 0; b < md->num_blocks; b++) {
+    size_t ti;
+    int    st = time_begin(md, NSM_KERNEL_ELEMENT, &ti);
+    if (st) return st;
+    if (md->precision == NSM_FP64)
+      nsm::launch_element_fp64(md->blk_off[b], md->blk_off[b + 1], md->conn, md->X, md->u, md->K[b], md->G[b],
+                               (double*)md->Fip, (double*)md->sip, (double*)md->slots, md->status, md->stream);
+    else
+      nsm::launch_element_mixed(md->blk_off[b], md->blk_off[b + 1], md->conn, md->X, md->u, (float)md->K[b],
+                                (float)md->G[b], (float*)md->Fip, (float*)md->sip, (float*)md->slots, md->status,
+                                md->stream);
+    NSM_LAUNCHED();
+    st = time_end(md, ti);
+    if (st) return st;
+  }
+  return NSM_OK;
+}
+
+int launch_node(nsm_model* md, nsm::StepTimes cur, bool output_bc, bool fuse, nsm::StepTimes nxt, bool write_a,
+                bool update_v)
+{
+  size_t ti;
+  int    st = time_begin(md, NSM_KERNEL_NODE, &ti);
+  if (st) return st;
+  const nsm::NodeArrays na = md->node_arrays();
+  if (md->precision == NSM_FP64)
+    nsm::launch_node_update<double>(na, (const double*)md->slots, cur, output_bc, fuse, nxt, write_a, update_v,
+                                    md->stream);
+  else
+    nsm::launch_node_update<float>(na, (const float*)md->slots, cur, output_bc, fuse, nxt, write_a, update_v,
+                                   md->stream);
+  NSM_LAUNCHED();
+  return time_end(md, ti);
+}
+
+int launch_reaction_at(nsm_model* md, int64_t step)
+{
+  if (md->num_reaction_nodes <= 0 || step < 0 || step >= md->reaction_cap) return NSM_OK;
+  size_t ti;
+  int    st = time_begin(md, NSM_KERNEL_REACTION, &ti);
+  if (st) return st;
+  nsm::launch_reaction(md->reaction_nodes, md->num_reaction_nodes, md->reaction_coord, md->f,
+                       md->reaction_hist + step, md->stream);
+  NSM_LAUNCHED();
+  return time_end(md, ti);
+}
+
+nsm::StepTimes step_times(double* time_current, double dt_user)
+{
+  // src/integrators/explicit_time_integrator.cc:192-195
+  nsm::StepTimes t;
+  t.t_prev  = *time_current;
+  *time_current += dt_user;
+  t.t_cur   = *time_current;
+  t.dt      = t.t_cur - t.t_prev;
+  t.half_dt = 0.5 * t.dt;
+  return t;
+}
+
+bool is_output(int64_t step, int64_t num_load_steps, int64_t output_frequency)
+{
+  if (output_frequency == 0) return false;
+  return step % output_frequency == 0 || step == num_load_steps - 1;
+}
+
+template <typename T>
+void free_dev(T*& p)
+{
+  if (p) (void)hipFree((void*)p);
+  p = nullptr;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* nsm_status_string(int status)
+{
+  switch (status) {
+    case NSM_OK: return "ok";
+    case NSM_ERR_ARG: return "invalid argument";
+    case NSM_ERR_HIP: return "HIP runtime error";
+    case NSM_ERR_NEG_JACOBIAN: return "non-positive Jacobian";
+    case NSM_ERR_STATE: return "invalid call order";
+    case NSM_ERR_NOMEM: return "out of device memory";
+    default: return "unknown status";
+  }
+}
+
+const char* nsm_last_error(void) { return g_last_error.c_str(); }
+
+const char* nsm_version(void) { return "nimblesm_amd 0.1 gfx950"; }
+
+// ------------------------------------------------------------------- seams
+#define NSM_SEAM_PROLOGUE(n)                                                          \
+  if ((n) < 0) return fail(NSM_ERR_ARG, "negative size");                             \
+  if ((n) == 0) return NSM_OK;                                                        \
+  int* status = nullptr;                                                              \
+  {                                                                                   \
+    int _st = seam_status_buffer(&status);                                            \
+    if (_st) return _st;                                                              \
+  }                                                                                   \
+  hipStream_t s = (hipStream_t)stream;                                                \
+  NSM_HIP(hipMemsetAsync(status, 0, sizeof(int), s));
+
+static int seam_epilogue(int* status, hipStream_t s)
+{
+  NSM_LAUNCHED();
+  int h = 0;
+  NSM_HIP(hipMemcpyAsync(&h, status, sizeof(int), hipMemcpyDeviceToHost, s));
+  NSM_HIP(hipStreamSynchronize(s));
+  if (h) return fail(NSM_ERR_NEG_JACOBIAN, "non-positive Jacobian determinant (Invert3x3)");
+  return NSM_OK;
+}
+
+int nsm_hex_compute_deformation_gradients(int64_t num_elem, const double* X_gathered, const double* u_gathered,
+                                          double* F, void* stream)
+{
+  if (num_elem > 0 && (!X_gathered || !u_gathered || !F)) return fail(NSM_ERR_ARG, "null pointer");
+  NSM_SEAM_PROLOGUE(num_elem);
+  nsm::launch_seam_defgrad(num_elem, X_gathered, u_gathered, F, status, s);
+  return seam_epilogue(status, s);
+}
+
+int nsm_hex_compute_nodal_forces(int64_t num_elem, const double* X_gathered, const double* u_gathered,
+                                 const double* sigma, double* f, void* stream)
+{
+  if (num_elem > 0 && (!X_gathered || !u_gathered || !sigma || !f)) return fail(NSM_ERR_ARG, "null pointer");
+  NSM_SEAM_PROLOGUE(num_elem);
+  nsm::launch_seam_forces(num_elem, X_gathered, u_gathered, sigma, f, status, s);
+  return seam_epilogue(status, s);
+}
+
+int nsm_hex_compute_lumped_mass(int64_t num_elem, double density, const double* X_gathered, double* m, void* stream)
+{
+  if (num_elem > 0 && (!X_gathered || !m)) return fail(NSM_ERR_ARG, "null pointer");
+  NSM_SEAM_PROLOGUE(num_elem);
+  nsm::launch_seam_lumped_mass(num_elem, density, X_gathered, m, status, s);
+  return seam_epilogue(status, s);
+}
+
+int nsm_hex_compute_characteristic_length(int64_t num_elem, const double* x_gathered, double* out, void* stream)
+{
+  if (num_elem > 0 && (!x_gathered || !out)) return fail(NSM_ERR_ARG, "null pointer");
+  if (num_elem < 0) return fail(NSM_ERR_ARG, "negative size");
+  if (num_elem == 0) return NSM_OK;
+  nsm::launch_seam_char_length(num_elem, x_gathered, out, (hipStream_t)stream);
+  NSM_LAUNCHED();
+  return NSM_OK;
+}
+
+int nsm_neohookean_get_stress(int64_t num_points, double bulk_modulus, double shear_modulus, const double* F_np1,
+                              double* sigma_np1, void* stream)
+{
+  if (num_points < 0) return fail(NSM_ERR_ARG, "negative size");
+  if (num_points == 0) return NSM_OK;
+  if (!F_np1 || !sigma_np1) return fail(NSM_ERR_ARG, "null pointer");
+  nsm::launch_seam_stress(num_points, bulk_modulus, shear_modulus, F_np1, sigma_np1, (hipStream_t)stream);
+  NSM_LAUNCHED();
+  return NSM_OK;
+}
+
+// ------------------------------------------------------------------- model
+int nsm_model_create(int64_t num_nodes, const double* reference_coordinates, int num_blocks, const int* block_ids,
+                     const int64_t* block_num_elem, const int* conn, const double* density,
+                     const double* bulk_modulus, const double* shear_modulus, int precision, nsm_model** out)
+{
+  if (!out) return fail(NSM_ERR_ARG, "out is null");
+  *out = nullptr;
+  if (num_nodes <= 0 || !reference_coordinates) return fail(NSM_ERR_ARG, "need at least one node");
+  if (num_blocks <= 0 || !block_ids || !block_num_elem || !density || !bulk_modulus || !shear_modulus)
+    return fail(NSM_ERR_ARG, "need at least one block with material parameters");
+  if (precision != NSM_FP64 && precision != NSM_MIXED) return fail(NSM_ERR_ARG, "unknown precision %d", precision);
+  if (num_nodes > (int64_t)INT32_MAX) return fail(NSM_ERR_ARG, "node ids are int32");
+
+  std::vector<int64_t> off(1, 0);
+  for (int b = 0; b < num_blocks; b++) {
+    if (block_num_elem[b] < 0) return fail(NSM_ERR_ARG, "negative element count in block %d", b);
+    if (!(density[b] > 0.0) || !(bulk_modulus[b] > 0.0) || !(shear_modulus[b] >= 0.0))
+      return fail(NSM_ERR_ARG, "invalid material parameters in block %d", b);
+    off.push_back(off.back() + block_num_elem[b]);
+  }
+  const int64_t E = off.back();
+  if (E > 0 && !conn) return fail(NSM_ERR_ARG, "conn is null");
+  if (8 * E >= (int64_t)UINT32_MAX) return fail(NSM_ERR_ARG, "too many elements for 32-bit CSR entries");
+  for (int64_t k = 0; k < 8 * E; k++)
+    if (conn[k] < 0 || conn[k] >= num_nodes)
+      return fail(NSM_ERR_ARG, "connectivity entry %lld = %d out of range", (long long)k, conn[k]);
+
+  // node -> (8*element + local node) incidence in ascending element order
+  std::vector<uint32_t> ptr((size_t)num_nodes + 1, 0), ent((size_t)(8 * E));
+  for (int64_t k = 0; k < 8 * E; k++) ptr[(size_t)conn[k] + 1]++;
+  for (int64_t i = 0; i < num_nodes; i++) ptr[(size_t)i + 1] += ptr[(size_t)i];
+  {
+    std::vector<uint32_t> fill(ptr.begin(), ptr.end() - 1);
+    for (int64_t k = 0; k < 8 * E; k++) ent[fill[(size_t)conn[k]]++] = (uint32_t)k;
+  }
+
+  nsm_model* md  = new nsm_model();
+  md->precision  = precision;
+  md->N          = num_nodes;
+  md->E          = E;
+  md->num_blocks = num_blocks;
+  md->block_ids.assign(block_ids, block_ids + num_blocks);
+  md->blk_off = off;
+  md->rho.assign(density, density + num_blocks);
+  md->K.assign(bulk_modulus, bulk_modulus + num_blocks);
+  md->G.assign(shear_modulus, shear_modulus + num_blocks);
+
+  auto cleanup = [&](int st) {
+    nsm_model_destroy(md);
+    return st;
+  };
+  if (hipGetDevice(&md->device) != hipSuccess) return cleanup(fail(NSM_ERR_HIP, "no HIP device"));
+  if (hipStreamCreateWithFlags(&md->stream, hipStreamNonBlocking) != hipSuccess)
+    return cleanup(fail(NSM_ERR_HIP, "hipStreamCreate failed"));
+
+  const size_t N3 = 3 * (size_t)num_nodes;
+  const size_t ipw = precision == NSM_FP64 ? sizeof(double) : sizeof(float);
+  size_t&      tot = md->device_bytes;
+  int          st  = 0;
+  if ((st = dalloc(&md->X, N3, &tot)) || (st = dalloc(&md->u, N3, &tot)) || (st = dalloc(&md->v, N3, &tot)) ||
+      (st = dalloc(&md->a, N3, &tot)) || (st = dalloc(&md->f, N3, &tot)) ||
+      (st = dalloc(&md->m, (size_t)num_nodes, &tot)) || (st = dalloc(&md->conn, (size_t)(8 * E), &tot)) ||
+      (st = dalloc(&md->node_ptr, ptr.size(), &tot)) || (st = dalloc(&md->node_ent, ent.size(), &tot)) ||
+      (st = dalloc((char**)&md->Fip, (size_t)(72 * E) * ipw, &tot)) ||
+      (st = dalloc((char**)&md->sip, (size_t)(48 * E) * ipw, &tot)) ||
+      (st = dalloc((char**)&md->slots, (size_t)(24 * E) * ipw, &tot)) ||
+      (st = dalloc(&md->status, 1, &tot)) || (st = dalloc(&md->bc_code, (size_t)num_nodes, &tot)))
+    return cleanup(st);
+
+  hipStream_t s = md->stream;
+#define NSM_HIP_C(call)                                                                            \
+  do {                                                                                             \
+    hipError_t _e = (call);                                                                        \
+    if (_e != hipSuccess) return cleanup(fail(NSM_ERR_HIP, "%s: %s", #call, hipGetErrorString(_e))); \
+  } while (0)
+  NSM_HIP_C(hipMemcpyAsync(md->X, reference_coordinates, N3 * sizeof(double), hipMemcpyHostToDevice, s));
+  if (E > 0) {
+    NSM_HIP_C(hipMemcpyAsync(md->conn, conn, (size_t)(8 * E) * sizeof(int), hipMemcpyHostToDevice, s));
+    NSM_HIP_C(hipMemcpyAsync(md->node_ent, ent.data(), ent.size() * sizeof(uint32_t), hipMemcpyHostToDevice, s));
+  }
+  NSM_HIP_C(hipMemcpyAsync(md->node_ptr, ptr.data(), ptr.size() * sizeof(uint32_t), hipMemcpyHostToDevice, s));
+  NSM_HIP_C(hipMemsetAsync(md->u, 0, N3 * sizeof(double), s));
+  NSM_HIP_C(hipMemsetAsync(md->v, 0, N3 * sizeof(double), s));
+  NSM_HIP_C(hipMemsetAsync(md->a, 0, N3 * sizeof(double), s));
+  NSM_HIP_C(hipMemsetAsync(md->f, 0, N3 * sizeof(double), s));
+  NSM_HIP_C(hipMemsetAsync(md->m, 0, (size_t)num_nodes * sizeof(double), s));
+  NSM_HIP_C(hipMemsetAsync(md->status, 0, sizeof(int), s));
+  NSM_HIP_C(hipMemsetAsync(md->bc_code, 0, (size_t)num_nodes, s));
+  if (E > 0) {
+    // STEP_NP1 deformation gradient starts at the identity, stress at zero
+    // (src/nimble_kokkos_model_data.cc:921-924)
+    std::vector<char> ident((size_t)(72 * E) * ipw);
+    for (int64_t p = 0; p < 8 * E; p++)
+      for (int c = 0; c < 9; c++) {
+        const double val = c < 3 ? 1.0 : 0.0;
+        if (precision == NSM_FP64)
+          ((double*)ident.data())[p * 9 + c] = val;
+        else
+          ((float*)ident.data())[p * 9 + c] = (float)val;
+      }
+    NSM_HIP_C(hipMemcpy(md->Fip, ident.data(), ident.size(), hipMemcpyHostToDevice));
+    NSM_HIP_C(hipMemsetAsync(md->sip, 0, (size_t)(48 * E) * ipw, s));
+  }
+  NSM_HIP_C(hipStreamSynchronize(s));
+#undef NSM_HIP_C
+  *out = md;
+  return NSM_OK;
+}
+
+int nsm_model_destroy(nsm_model* md)
+{
+  if (!md) return NSM_OK;
+  (void)hipSetDevice(md->device);
+  if (md->stream) (void)hipStreamSynchronize(md->stream);
+  free_dev(md->X);
+  free_dev(md->u);
+  free_dev(md->v);
+  free_dev(md->a);
+  free_dev(md->f);
+  free_dev(md->m);
+  free_dev(md->conn);
+  free_dev(md->node_ptr);
+  free_dev(md->node_ent);
+  free_dev(md->Fip);
+  free_dev(md->sip);
+  free_dev(md->slots);
+  free_dev(md->elem_tmp);
+  free_dev(md->wg_min);
+  free_dev(md->status);
+  free_dev(md->bc_code);
+  free_dev(md->bc_table);
+  free_dev(md->reaction_nodes);
+  free_dev(md->reaction_hist);
+  for (auto& p : md->event_pool) {
+    (void)hipEventDestroy(p.start);
+    (void)hipEventDestroy(p.stop);
+  }
+  if (md->stream) (void)hipStreamDestroy(md->stream);
+  delete md;
+  return NSM_OK;
+}
+
+int nsm_model_add_boundary_condition(nsm_model* md, int kind, int coordinate, double magnitude, int64_t num_nodes,
+                                     const int* node_ids)
+{
+  if (!md) return fail(NSM_ERR_ARG, "null model");
+  if (kind < NSM_BC_INITIAL_VELOCITY || kind > NSM_BC_PRESCRIBED_DISPLACEMENT)
+    return fail(NSM_ERR_ARG, "unknown boundary-condition kind %d", kind);
+  if (coordinate < 0 || coordinate > 2) return fail(NSM_ERR_ARG, "coordinate must be 0, 1 or 2");
+  if (num_nodes < 0 || (num_nodes > 0 && !node_ids)) return fail(NSM_ERR_ARG, "invalid node list");
+  BcRecord r;
+  r.kind  = kind;
+  r.coord = coordinate;
+  r.value = magnitude;
+  r.nodes.assign(node_ids, node_ids + num_nodes);
+  for (int n : r.nodes)
+    if (n < 0 || n >= md->N) return fail(NSM_ERR_ARG, "node id %d out of range", n);
+  md->bcs.push_back(std::move(r));
+  md->bc_dirty = true;
+  return NSM_OK;
+}
+
+int nsm_model_apply_initial_conditions(nsm_model* md)
+{
+  if (!md) return fail(NSM_ERR_ARG, "null model");
+  int st = ensure_device(md);
+  if (st) return st;
+  std::vector<double> v((size_t)(3 * md->N));
+  NSM_HIP(hipMemcpyAsync(v.data(), md->v, v.size() * sizeof(double), hipMemcpyDeviceToHost, md->stream));
+  NSM_HIP(hipStreamSynchronize(md->stream));
+  for (const auto& b : md->bcs)
+    if (b.kind == NSM_BC_INITIAL_VELOCITY)
+      for (int n : b.nodes) v[(size_t)(3 * (int64_t)n + b.coord)] = b.value;
+  NSM_HIP(hipMemcpyAsync(md->v, v.data(), v.size() * sizeof(double), hipMemcpyHostToDevice, md->stream));
+  NSM_HIP(hipStreamSynchronize(md->stream));
+  return NSM_OK;
+}
+
+int nsm_model_apply_kinematic_conditions(nsm_model* md, double time_current, double time_previous)
+{
+  if (!md) return fail(NSM_ERR_ARG, "null model");
+  int st = ensure_device(md);
+  if (st || (st = rebuild_bcs(md))) return st;
+  nsm::StepTimes t{time_previous, time_current, time_current - time_previous, 0.0};
+  nsm::launch_apply_bc(md->node_arrays(), t, md->stream);
+  NSM_LAUNCHED();
+  return NSM_OK;
+}
+
+int nsm_model_compute_lumped_mass(nsm_model* md)
+{
+  if (!md) return fail(NSM_ERR_ARG, "null model");
+  int st = ensure_device(md);
+  if (st) return st;
+  size_t tmp = 0;
+  if (!md->elem_tmp && (st = dalloc(&md->elem_tmp, std::max<size_t>((size_t)(8 * md->E), 1), &tmp))) return st;
+  md->device_bytes += tmp;
+  for (int b = 0; b < md->num_blocks; b++) {
+    nsm::launch_element_mass(md->blk_off[b], md->blk_off[b + 1], md->conn, md->X, md->rho[b], md->elem_tmp,
+                             md->status, md->stream);
+    NSM_LAUNCHED();
+  }
+  nsm::launch_mass_gather(md->N, md->node_ptr, md->node_ent, md->elem_tmp, md->m, md->stream);
+  NSM_LAUNCHED();
+
+  // critical time step, minimum over blocks of min_e charlen_e / sqrt(K/rho)
+  int64_t need = 0;
+  for (int b = 0; b < md->num_blocks; b++)
+    need = std::max<int64_t>(need, (md->blk_off[b + 1] - md->blk_off[b] + 255) / 256 + 1);
+  if (need > md->wg_min_len) {
+    free_dev(md->wg_min);
+    tmp = 0;
+    if ((st = dalloc(&md->wg_min, (size_t)need, &tmp))) return st;
+    md->wg_min_len = need;
+  }
+  double dt = 1.7976931348623157e308;
+  for (int b = 0; b < md->num_blocks; b++) {
+    const double c  = std::sqrt(md->K[b] / md->rho[b]);
+    const int    ng = nsm::launch_element_dt(md->blk_off[b], md->blk_off[b + 1], md->conn, md->X, md->u, c,
+                                             md->wg_min, md->stream);
+    NSM_LAUNCHED();
+    if (ng == 0) continue;
+    nsm::launch_min_reduce(md->wg_min, ng, md->wg_min + md->wg_min_len - 1, md->stream);
+    NSM_LAUNCHED();
+    double bdt = 0.0;
+    NSM_HIP(hipMemcpyAsync(&bdt, md->wg_min + md->wg_min_len - 1, sizeof(double), hipMemcpyDeviceToHost,
+                           md->stream));
+    NSM_HIP(hipStreamSynchronize(md->stream));
+    if (bdt < dt) dt = bdt;
+  }
+  md->critical_dt = dt;
+  if ((st = check_status(md))) return st;
+  md->have_mass = true;
+  return NSM_OK;
+}
+
+int nsm_model_get_critical_time_step(nsm_model* md, double* dt)
+{
+  if (!md || !dt) return fail(NSM_ERR_ARG, "null argument");
+  if (!md->have_mass) return fail(NSM_ERR_STATE, "call nsm_model_compute_lumped_mass first");
+  *dt = md->critical_dt;
+  return NSM_OK;
+}
+
+int nsm_model_compute_internal_force(nsm_model* md, double time_previous, double time_current, int is_output_step)
+{
+  (void)time_previous;
+  (void)time_current;
+  (void)is_output_step;
+  if (!md) return fail(NSM_ERR_ARG, "null model");
+  int st = ensure_device(md);
+  if (st || (st = launch_elements(md))) return st;
+  const nsm::NodeArrays na = md->node_arrays();
+  if (md->precision == NSM_FP64)
+    nsm::launch_force_gather<double>(na, (const double*)md->slots, md->stream);
+  else
+    nsm::launch_force_gather<float>(na, (const float*)md->slots, md->stream);
+  NSM_LAUNCHED();
+  return check_status(md);
+}
+
+int nsm_model_update_states(nsm_model* md)
+{
+  if (!md) return fail(NSM_ERR_ARG, "null model");
+  return NSM_OK;
+}
+
+int nsm_model_set_reaction_nodes(nsm_model* md, int64_t num_nodes, const int* node_ids, int coordinate)
+{
+  if (!md) return fail(NSM_ERR_ARG, "null model");
+  if (coordinate < 0 || coordinate > 2) return fail(NSM_ERR_ARG, "coordinate must be 0, 1 or 2");
+  if (num_nodes < 0 || (num_nodes > 0 && !node_ids)) return fail(NSM_ERR_ARG, "invalid node list");
+  for (int64_t k = 0; k < num_nodes; k++)
+    if (node_ids[k] < 0 || node_ids[k] >= md->N) return fail(NSM_ERR_ARG, "node id %d out of range", node_ids[k]);
+  int st = ensure_device(md);
+  if (st) return st;
+  free_dev(md->reaction_nodes);
+  md->num_reaction_nodes = 0;
+  if (num_nodes > 0) {
+    size_t tmp = 0;
+    if ((st = dalloc(&md->reaction_nodes, (size_t)num_nodes, &tmp))) return st;
+    NSM_HIP(hipMemcpy(md->reaction_nodes, node_ids, (size_t)num_nodes * sizeof(int), hipMemcpyHostToDevice));
+  }
+  md->num_reaction_nodes = num_nodes;
+  md->reaction_coord     = coordinate;
+  return NSM_OK;
+}
+
+int nsm_model_get_reaction_history(nsm_model* md, int64_t first_step, int64_t count, double* out)
+{
+  if (!md || (count > 0 && !out)) return fail(NSM_ERR_ARG, "null argument");
+  if (first_step < 0 || count < 0 || first_step + count > md->reaction_cap)
+    return fail(NSM_ERR_ARG, "reaction history range [%lld, %lld) not recorded", (long long)first_step,
+                (long long)(first_step + count));
+  if (count == 0) return NSM_OK;
+  int st = ensure_device(md);
+  if (st) return st;
+  NSM_HIP(hipMemcpyAsync(out, md->reaction_hist + first_step, (size_t)count * sizeof(double), hipMemcpyDeviceToHost,
+                         md->stream));
+  NSM_HIP(hipStreamSynchronize(md->stream));
+  return NSM_OK;
+}
+
+int nsm_model_explicit_steps(nsm_model* md, int64_t step_begin, int64_t step_end, int64_t num_load_steps,
+                             int64_t output_frequency, double dt_user, double* time_current)
+{
+  if (!md || !time_current) return fail(NSM_ERR_ARG, "null argument");
+  if (step_begin < 0 || step_end < step_begin || step_end > num_load_steps)
+    return fail(NSM_ERR_ARG, "invalid step range");
+  if (!md->have_mass) return fail(NSM_ERR_STATE, "call nsm_model_compute_lumped_mass first");
+  if (step_end == step_begin) return NSM_OK;
+  int st = ensure_device(md);
+  if (st || (st = rebuild_bcs(md))) return st;
+  if (md->num_reaction_nodes > 0 && num_load_steps > md->reaction_cap) {
+    double* h   = nullptr;
+    size_t  tmp = 0;
+    if ((st = dalloc(&h, (size_t)num_load_steps, &tmp))) return st;
+    NSM_HIP(hipMemsetAsync(h, 0, (size_t)num_load_steps * sizeof(double), md->stream));
+    if (md->reaction_hist && md->reaction_cap > 0)
+      NSM_HIP(hipMemcpyAsync(h, md->reaction_hist, (size_t)md->reaction_cap * sizeof(double),
+                             hipMemcpyDeviceToDevice, md->stream));
+    NSM_HIP(hipStreamSynchronize(md->stream));
+    free_dev(md->reaction_hist);
+    md->reaction_hist = h;
+    md->reaction_cap  = num_load_steps;
+  }
+
+  double         t   = *time_current;
+  nsm::StepTimes cur = step_times(&t, dt_user);
+  nsm::launch_prekick(md->node_arrays(), cur, md->stream);
+  NSM_LAUNCHED();
+  for (int64_t step = step_begin; step < step_end; step++) {
+    if ((st = launch_elements(md))) return st;
+    const bool     last = step + 1 == step_end;
+    double         tn   = t;
+    nsm::StepTimes nxt  = cur;
+    if (!last) nxt = step_times(&tn, dt_user);
+    if ((st = launch_node(md, cur, is_output(step, num_load_steps, output_frequency), !last, nxt, last, true)))
+      return st;
+    if ((st = launch_reaction_at(md, step))) return st;
+    if (!last) {
+      t   = tn;
+      cur = nxt;
+    }
+  }
+  *time_current = t;
+  return check_status(md);
+}
+
+int nsm_model_get_field(nsm_model* md, int field, double* out)
+{
+  if (!md || !out) return fail(NSM_ERR_ARG, "null argument");
+  int st = ensure_device(md);
+  if (st) return st;
+  const size_t N3 = 3 * (size_t)md->N;
+  const double* src = nullptr;
+  size_t        cnt = 0;
+  switch (field) {
+    case NSM_FIELD_REFERENCE_COORDINATE: src = md->X, cnt = N3; break;
+    case NSM_FIELD_DISPLACEMENT: src = md->u, cnt = N3; break;
+    case NSM_FIELD_VELOCITY: src = md->v, cnt = N3; break;
+    case NSM_FIELD_ACCELERATION: src = md->a, cnt = N3; break;
+    case NSM_FIELD_INTERNAL_FORCE: src = md->f, cnt = N3; break;
+    case NSM_FIELD_LUMPED_MASS: src = md->m, cnt = (size_t)md->N; break;
+    case NSM_FIELD_DEFORMATION_GRADIENT:
+    case NSM_FIELD_STRESS: {
+      const size_t per = field == NSM_FIELD_DEFORMATION_GRADIENT ? 72 : 48;
+      const void*  p   = field == NSM_FIELD_DEFORMATION_GRADIENT ? md->Fip : md->sip;
+      cnt              = per * (size_t)md->E;
+      if (cnt == 0) return NSM_OK;
+      if (md->precision == NSM_FP64) {
+        NSM_HIP(hipMemcpyAsync(out, p, cnt * sizeof(double), hipMemcpyDeviceToHost, md->stream));
+      } else {
+        std::vector<float> tmp(cnt);
+        NSM_HIP(hipMemcpyAsync(tmp.data(), p, cnt * sizeof(float), hipMemcpyDeviceToHost, md->stream));
+        NSM_HIP(hipStreamSynchronize(md->stream));
+        for (size_t i = 0; i < cnt; i++) out[i] = tmp[i];
+      }
+      NSM_HIP(hipStreamSynchronize(md->stream));
+      return NSM_OK;
+    }
+    default: return fail(NSM_ERR_ARG, "unknown field %d", field);
+  }
+  NSM_HIP(hipMemcpyAsync(out, src, cnt * sizeof(double), hipMemcpyDeviceToHost, md->stream));
+  NSM_HIP(hipStreamSynchronize(md->stream));
+  return NSM_OK;
+}
+
+int nsm_model_set_field(nsm_model* md, int field, const double* in)
+{
+  if (!md || !in) return fail(NSM_ERR_ARG, "null argument");
+  int st = ensure_device(md);
+  if (st) return st;
+  const size_t N3  = 3 * (size_t)md->N;
+  double*      dst = nullptr;
+  size_t       cnt = N3;
+  switch (field) {
+    case NSM_FIELD_DISPLACEMENT: dst = md->u; break;
+    case NSM_FIELD_VELOCITY: dst = md->v; break;
+    case NSM_FIELD_ACCELERATION: dst = md->a; break;
+    case NSM_FIELD_INTERNAL_FORCE: dst = md->f; break;
+    case NSM_FIELD_LUMPED_MASS:
+      dst           = md->m;
+      cnt           = (size_t)md->N;
+      md->have_mass = true;
+      break;
+    default: return fail(NSM_ERR_ARG, "field %d cannot be set", field);
+  }
+  NSM_HIP(hipMemcpyAsync(dst, in, cnt * sizeof(double), hipMemcpyHostToDevice, md->stream));
+  NSM_HIP(hipStreamSynchronize(md->stream));
+  return NSM_OK;
+}
+
+int nsm_model_field_device_pointer(nsm_model* md, int field, void** ptr, int* elem_bytes)
+{
+  if (!md || !ptr) return fail(NSM_ERR_ARG, "null argument");
+  int eb = 8;
+  switch (field) {
+    case NSM_FIELD_REFERENCE_COORDINATE: *ptr = md->X; break;
+    case NSM_FIELD_DISPLACEMENT: *ptr = md->u; break;
+    case NSM_FIELD_VELOCITY: *ptr = md->v; break;
+    case NSM_FIELD_ACCELERATION: *ptr = md->a; break;
+    case NSM_FIELD_INTERNAL_FORCE: *ptr = md->f; break;
+    case NSM_FIELD_LUMPED_MASS: *ptr = md->m; break;
+    case NSM_FIELD_DEFORMATION_GRADIENT:
+      *ptr = md->Fip;
+      eb   = md->precision == NSM_FP64 ? 8 : 4;
+      break;
+    case NSM_FIELD_STRESS:
+      *ptr = md->sip;
+      eb   = md->precision == NSM_FP64 ? 8 : 4;
+      break;
+    default: return fail(NSM_ERR_ARG, "unknown field %d", field);
+  }
+  if (elem_bytes) *elem_bytes = eb;
+  return NSM_OK;
+}
+
+int nsm_model_synchronize(nsm_model* md)
+{
+  if (!md) return fail(NSM_ERR_ARG, "null model");
+  NSM_HIP(hipSetDevice(md->device));
+  NSM_HIP(hipStreamSynchronize(md->stream));
+  return NSM_OK;
+}
+
+int nsm_model_enable_kernel_timing(nsm_model* md, int enable)
+{
+  if (!md) return fail(NSM_ERR_ARG, "null model");
+  NSM_HIP(hipStreamSynchronize(md->stream));
+  md->timing     = enable != 0;
+  md->event_next = 0;
+  md->timed.clear();
+  return NSM_OK;
+}
+
+int nsm_model_kernel_time(nsm_model* md, int kernel, double* avg_ms, int64_t* launches)
+{
+  if (!md || !avg_ms) return fail(NSM_ERR_ARG, "null argument");
+  NSM_HIP(hipStreamSynchronize(md->stream));
+  double  sum = 0.0;
+  int64_t n   = 0;
+  for (const auto& t : md->timed) {
+    if (t.first != kernel) continue;
+    float ms = 0.f;
+    NSM_HIP(hipEventElapsedTime(&ms, md->event_pool[t.second].start, md->event_pool[t.second].stop));
+    sum += ms;
+    n++;
+  }
+  *avg_ms = n ? sum / (double)n : 0.0;
+  if (launches) *launches = n;
+  return NSM_OK;
+}
+
+int nsm_model_info(nsm_model* md, int64_t* num_nodes, int64_t* num_elem, int64_t* csr_entries, int64_t* device_bytes)
+{
+  if (!md) return fail(NSM_ERR_ARG, "null model");
+  if (num_nodes) *num_nodes = md->N;
+  if (num_elem) *num_elem = md->E;
+  if (csr_entries) *csr_entries = 8 * md->E;
+  if (device_bytes) *device_bytes = (int64_t)md->device_bytes;
+  return NSM_OK;
+}
+
+// ------------------------------------------------------------ synthetic mesh
+static uint64_t splitmix64(uint64_t x)
+{
+  uint64_t z = x + 0x9E3779B97F4A7C15ULL;
+  z          = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+  z          = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+  return z ^ (z >> 31);
+}
+
+int nsm_jittered_cube(int n, uint64_t seed, double jitter_fraction, double* X, int* conn)
+{
+  if (n <= 0 || !X || !conn) return fail(NSM_ERR_ARG, "invalid cube arguments");
+  const int64_t n1 = n + 1;
+  if (n1 * n1 * n1 > (int64_t)INT32_MAX) return fail(NSM_ERR_ARG, "cube too large for int32 node ids");
+  const uint64_t key = splitmix64(seed);
+  for (int64_t k = 0; k <= n; k++)
+    for (int64_t j = 0; j <= n; j++)
+      for (int64_t i = 0; i <= n; i++) {
+        const int64_t id = i + n1 * (j + n1 * k);
+        X[3 * id]        = (double)i / (double)n;
+        X[3 * id + 1]    = (double)j / (double)n;
+        X[3 * id + 2]    = (double)k / (double)n;
+        if (i > 0 && i < n && j > 0 && j < n && k > 0 && k < n)
+          for (int d = 0; d < 3; d++) {
+            const uint64_t r = splitmix64(key ^ (uint64_t)(3 * id + d));
+            const double   U = (double)(r >> 11) * (1.0 / 9007199254740992.0);
+            X[3 * id + d] += (2.0 * U - 1.0) * jitter_fraction / (double)n;
+          }
+      }
+  for (int64_t k = 0; k < n; k++)
+    for (int64_t j = 0; j < n; j++)
+      for (int64_t i = 0; i < n; i++) {
+        const int64_t e  = i + n * (j + n * k);
+        const int64_t b  = i + n1 * (j + n1 * k);
+        int*          c  = conn + 8 * e;
+        c[0]             = (int)b;
+        c[1]             = (int)(b + 1);
+        c[2]             = (int)(b + 1 + n1);
+        c[3]             = (int)(b + n1);
+        c[4]             = (int)(b + n1 * n1);
+        c[5]             = (int)(b + 1 + n1 * n1);
+        c[6]             = (int)(b + 1 + n1 + n1 * n1);
+        c[7]             = (int)(b + n1 + n1 * n1);
+      }
+  return NSM_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,99 @@
+"""The C ABI library loads and exports every entry point that
+include/nimblesm_amd.h declares; host-only entry points behave (no GPU
+compute is issued here)."""
+import ctypes as C
+import os
+import re
+
+import numpy as np
+
+import nimblesm_amd as N
+from nimblesm_amd import _capi
+from oracle import oracle as O
+
+HEADER = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "include", "nimblesm_amd.h")
+
+
+def declared_symbols():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(nsm_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_header_declares_the_entry_points():
+    syms = declared_symbols()
+    for s in ("nsm_hex_compute_deformation_gradients", "nsm_hex_compute_nodal_forces", "nsm_neohookean_get_stress",
+              "nsm_model_create", "nsm_model_compute_internal_force", "nsm_model_explicit_steps"):
+        assert s in syms
+
+
+def test_library_exports_every_declared_symbol():
+    lib = N.lib()
+    for s in declared_symbols():
+        assert hasattr(lib, s), s
+    assert set(declared_symbols()) == set(_capi.SIGNATURES), "ctypes signatures out of sync with the header"
+
+
+def test_version_and_status_strings():
+    lib = N.lib()
+    assert b"gfx950" in lib.nsm_version()
+    assert lib.nsm_status_string(0) == b"ok"
+    assert lib.nsm_status_string(3) == b"non-positive Jacobian"
+
+
+def test_jittered_cube_matches_oracle():
+    for n, seed in [(1, 0), (3, 99), (16, 1234)]:
+        X, conn = N.jittered_cube(n, seed, 0.1)
+        Xo, co = O.jittered_cube(n, seed, 0.1)
+        assert np.array_equal(X, Xo) and np.array_equal(conn, co)
+    X, conn = N.jittered_cube(8, 5, 0.1)
+    boundary = (X == 0.0) | (X == 1.0)
+    assert np.all(boundary.any(axis=1) | ((X > 0) & (X < 1)).all(axis=1))
+    assert np.max(np.abs(X * 8 - np.round(X * 8))) <= 0.1 + 1e-12
+
+
+def test_argument_validation_without_device():
+    lib = N.lib()
+    h = C.c_void_p()
+    X = np.zeros((8, 3))
+    conn = np.arange(8, dtype=np.int32).reshape(1, 8)
+    conn_bad = conn.copy()
+    conn_bad[0, 3] = 8
+    ids = np.array([1], np.int32)
+    ne = np.array([1], np.int64)
+    p = np.array([1.0])
+    st = lib.nsm_model_create(8, X.ctypes.data, 1, ids.ctypes.data, ne.ctypes.data, conn_bad.ctypes.data,
+                              p.ctypes.data, p.ctypes.data, p.ctypes.data, 0, C.byref(h))
+    assert st == _capi.NSM_ERR_ARG and b"out of range" in lib.nsm_last_error()
+    st = lib.nsm_model_create(8, X.ctypes.data, 1, ids.ctypes.data, ne.ctypes.data, conn.ctypes.data,
+                              p.ctypes.data, p.ctypes.data, p.ctypes.data, 7, C.byref(h))
+    assert st == _capi.NSM_ERR_ARG and b"precision" in lib.nsm_last_error()
+    neg = np.array([-1.0])
+    st = lib.nsm_model_create(8, X.ctypes.data, 1, ids.ctypes.data, ne.ctypes.data, conn.ctypes.data,
+                              neg.ctypes.data, p.ctypes.data, p.ctypes.data, 0, C.byref(h))
+    assert st == _capi.NSM_ERR_ARG
+    assert lib.nsm_jittered_cube(0, 1, 0.1, X.ctypes.data, conn.ctypes.data) == _capi.NSM_ERR_ARG
+    assert lib.nsm_neohookean_get_stress(0, 1.0, 1.0, None, None, None) == 0  # empty is a no-op
+    assert lib.nsm_neohookean_get_stress(-1, 1.0, 1.0, None, None, None) == _capi.NSM_ERR_ARG
+
+
+def test_problem_definitions():
+    p = N.workload("cube128_2blk")
+    assert p.num_elem == 2_097_152 and p.num_nodes == 2_146_689
+    assert p.block_ranges() == [(0, 64 * 128 * 128), (64 * 128 * 128, 128 ** 3)]
+    q = N.workload("cube256_4blk")
+    assert [b - a for a, b in q.block_ranges()] == [256 * 256 * 64] * 4
+    s = N.workload("cube16_fp64")
+    assert len(s.top) == 17 * 17 and s.top.min() == 16 * 17 * 17
+
+
+def test_single_hip_runtime_in_process():
+    """Loading the library alongside torch must leave exactly one HIP / HSA
+    runtime mapped (two runtimes cannot share a device)."""
+    N.lib()
+    import torch  # noqa: F401
+
+    maps = open("/proc/self/maps").read()
+    hip = set(re.findall(r"(/\S*libamdhip64\S*)", maps))
+    hsa = set(re.findall(r"(/\S*libhsa-runtime64\S*)", maps))
+    assert len(hip) == 1 and len(hsa) == 1, (hip, hsa)

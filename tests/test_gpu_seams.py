@@ -1,0 +1,105 @@
+"""GPU element / material seams against the oracle (FP64, reference order).
+
+The HIP kernels reproduce NimbleSM's operation order with -ffp-contract=off;
+everything but the stress is expected bit-exact.  The stress goes through
+cbrt, whose last bit may differ between the device math library and glibc,
+so it is held to 1e-14 relative per point."""
+import numpy as np
+import pytest
+
+import nimblesm_amd as N
+from oracle import oracle as O
+from tests.helpers import random_elements
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def elems():
+    return random_elements(4099, seed=21)  # not a multiple of the 32-element workgroup
+
+
+def _dev(a, device):
+    import torch
+
+    return torch.from_numpy(np.ascontiguousarray(a)).to(device)
+
+
+def test_deformation_gradients_bit_exact(hip_device, elems):
+    import torch
+
+    X, u = elems
+    F = torch.zeros((len(X), 8, 9), dtype=torch.float64, device=hip_device)
+    N.HexElement.ComputeDeformationGradients(_dev(X, hip_device), _dev(u, hip_device), F)
+    torch.cuda.synchronize()
+    assert np.array_equal(F.cpu().numpy(), O.deformation_gradients(X, u))
+
+
+def test_stress_matches_oracle(hip_device, elems):
+    import torch
+
+    X, u = elems
+    Fo = O.deformation_gradients(X, u)
+    for K, G in [(166.67e9, 76.92e9), (1.6e12, 0.8e12), (10.0e9, 1.034e9)]:
+        mat = N.NeohookeanMaterial(7800.0, K, G)
+        s = torch.zeros((len(X), 8, 6), dtype=torch.float64, device=hip_device)
+        mat.GetStress(_dev(Fo, hip_device), s)
+        torch.cuda.synchronize()
+        sg, so = s.cpu().numpy().reshape(-1, 6), O.stress(K, G, Fo).reshape(-1, 6)
+        err = np.abs(sg - so).max(axis=1) / np.maximum(np.abs(so).max(axis=1), 1e-300)
+        assert err.max() <= 1e-14, err.max()
+        assert np.mean(np.all(sg == so, axis=1)) > 0.5  # most points bit-identical
+
+
+def test_identity_and_zero_sized(hip_device):
+    import torch
+
+    F = torch.zeros((3, 9), dtype=torch.float64, device=hip_device)
+    F[:, :3] = 1.0
+    s = torch.full((3, 6), 7.0, dtype=torch.float64, device=hip_device)
+    N.NeohookeanMaterial(1.0, 1e9, 1e9).GetStress(F, s)
+    torch.cuda.synchronize()
+    assert torch.all(s == 0)
+    e = torch.zeros((0, 8, 3), dtype=torch.float64, device=hip_device)
+    N.HexElement.ComputeDeformationGradients(e, e, torch.zeros((0, 8, 9), dtype=torch.float64, device=hip_device))
+
+
+def test_nodal_forces_bit_exact(hip_device, elems):
+    import torch
+
+    X, u = elems
+    so = O.stress(1.6e11, 0.8e11, O.deformation_gradients(X, u))
+    f = torch.zeros((len(X), 8, 3), dtype=torch.float64, device=hip_device)
+    N.HexElement.ComputeNodalForces(_dev(X, hip_device), _dev(u, hip_device), _dev(so, hip_device), f)
+    torch.cuda.synchronize()
+    assert np.array_equal(f.cpu().numpy(), O.nodal_forces(X, u, so))
+
+
+def test_lumped_mass_and_char_length_bit_exact(hip_device, elems):
+    import torch
+
+    X, u = elems
+    m = torch.zeros((len(X), 8), dtype=torch.float64, device=hip_device)
+    N.HexElement.ComputeLumpedMass(7800.0, _dev(X, hip_device), m)
+    cl = torch.zeros(len(X), dtype=torch.float64, device=hip_device)
+    x = X + u
+    N.HexElement.ComputeCharacteristicLength(_dev(x, hip_device), cl)
+    torch.cuda.synchronize()
+    mo = np.array([O.lumped_mass_elem(7800.0, X[e]) for e in range(len(X))])
+    assert np.array_equal(m.cpu().numpy(), mo)
+    assert np.array_equal(cl.cpu().numpy(), np.array([O.char_length(x[e]) for e in range(len(x))]))
+
+
+def test_inverted_element_reports_error(hip_device):
+    import torch
+
+    X = np.array([[0, 0, 0], [1, 0, 0], [1, 1, 0], [0, 1, 0], [0, 0, 1], [1, 0, 1], [1, 1, 1], [0, 1, 1]], float)
+    u = np.zeros_like(X)
+    u[4:, 2] = -2.0  # top face pushed below the bottom face: det J < 0
+    F = torch.zeros((1, 8, 9), dtype=torch.float64, device=hip_device)
+    with pytest.raises(N.NimbleError) as ei:
+        N.HexElement.ComputeNodalForces(_dev(X[None], hip_device), _dev(u[None], hip_device),
+                                        torch.zeros((1, 8, 6), dtype=torch.float64, device=hip_device),
+                                        torch.zeros((1, 8, 3), dtype=torch.float64, device=hip_device))
+    assert ei.value.status == 3
+    N.HexElement.ComputeDeformationGradients(_dev(X[None], hip_device), _dev(np.zeros_like(u)[None], hip_device), F)

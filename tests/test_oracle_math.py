@@ -1,0 +1,192 @@
+"""Checks of the oracle that follow from the mathematics alone (closed forms,
+invariants, special cases), chosen so that a dropped term, a wrong sign or a
+transposed operand fails at least one of them."""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from tests.helpers import UNIT_HEX, kf_to_mat, mat_to_kf, random_elements, sym_to_mat
+
+K, G = 166.67e9, 76.92e9
+
+
+def closed_form_stress(F, K, G):
+    """sigma = K/2 (J - 1/J) I + G/J dev(J^{-2/3} F F^T) for one F (3x3)."""
+    J = np.linalg.det(F)
+    B = F @ F.T * J ** (-2.0 / 3.0)
+    dev = B - np.trace(B) / 3.0 * np.eye(3)
+    return 0.5 * K * (J - 1.0 / J) * np.eye(3) + G / J * dev
+
+
+def test_gauss_tables():
+    ip, N, dN = O.hex_tables()
+    assert np.all(np.abs(np.abs(ip) - 0.577350269189626) == 0)
+    assert np.allclose(N.sum(axis=1), 1.0, atol=1e-15)
+    assert np.allclose(dN.sum(axis=1), 0.0, atol=1e-16)
+    # N_a(xi_q) is the trilinear basis: interpolates node positions exactly
+    xq = N @ UNIT_HEX
+    assert np.allclose(xq, (ip + 1.0) / 2.0, atol=1e-15)
+
+
+def test_pade_is_cos_acos_over_3():
+    x = np.linspace(0.0, 1.0, 2001)
+    approx = np.array([O.lib().orc_cos_of_acos_div3(t) for t in x])
+    assert np.max(np.abs(approx - np.cos(np.arccos(x) / 3.0))) < 1e-15
+
+
+def test_polar_decomposition_invariants():
+    X, u = random_elements(400, seed=5)
+    F = O.deformation_gradients(X, u).reshape(-1, 9)
+    for p in range(0, len(F), 3):
+        Fm = kf_to_mat(F[p])
+        V, R = O.polar_decomp(F[p])
+        Vm, Rm = sym_to_mat(V), kf_to_mat(R)
+        assert np.linalg.norm(Vm @ Rm - Fm) < 1e-12  # CheckCorrectnessOfPolarDecomp tolerance
+        assert np.allclose(Rm @ Rm.T, np.eye(3), atol=1e-12)
+        assert np.all(np.linalg.eigvalsh(Vm) > 0)
+        assert np.allclose(Vm @ Vm, Fm @ Fm.T, rtol=0, atol=1e-12 * np.linalg.norm(Fm) ** 2)
+
+
+def test_polar_special_cases():
+    V, R = O.polar_decomp(mat_to_kf(np.eye(3)))
+    assert np.array_equal(V, [1, 1, 1, 0, 0, 0]) and np.array_equal(R, mat_to_kf(np.eye(3)))
+    D = np.diag([1.2, 0.9, 1.05])
+    V, _ = O.polar_decomp(mat_to_kf(D))
+    assert np.allclose(sym_to_mat(V), D, atol=1e-15)
+    th = 0.3
+    Rz = np.array([[np.cos(th), -np.sin(th), 0], [np.sin(th), np.cos(th), 0], [0, 0, 1]])
+    V, R = O.polar_decomp(mat_to_kf(Rz @ D))  # F = R U  ->  V = R U R^T
+    assert np.allclose(sym_to_mat(V), Rz @ D @ Rz.T, atol=1e-14)
+    assert np.allclose(kf_to_mat(R), Rz, atol=1e-14)
+
+
+def test_stress_closed_form():
+    X, u = random_elements(300, seed=6, amps=(1e-3, 0.03))
+    F = O.deformation_gradients(X, u).reshape(-1, 9)
+    s = O.stress(K, G, F)
+    for p in range(0, len(F), 5):
+        ref = closed_form_stress(kf_to_mat(F[p]), K, G)
+        assert np.linalg.norm(sym_to_mat(s[p]) - ref) <= 1e-11 * np.linalg.norm(ref)
+
+
+def test_stress_special_cases():
+    assert np.array_equal(O.stress(K, G, mat_to_kf(np.eye(3))), np.zeros(6))
+    # pure rotation: stress-free
+    th = 0.7
+    Rz = np.array([[np.cos(th), -np.sin(th), 0], [np.sin(th), np.cos(th), 0], [0, 0, 1]])
+    assert np.max(np.abs(O.stress(K, G, mat_to_kf(Rz)))) < 1e-4  # ~1e-15 relative to K
+    # uniform dilation lam: sigma = K/2 (J - 1/J) I, J = lam^3, no deviator
+    lam = 1.01
+    s = O.stress(K, G, mat_to_kf(lam * np.eye(3)))
+    J = lam ** 3
+    assert np.allclose(s[:3], 0.5 * K * (J - 1 / J), rtol=1e-14) and np.allclose(s[3:], 0, atol=1e-3)
+    # simple shear gamma: sigma_xy = G * gamma / J^{5/3} with J = 1
+    g = 1e-3
+    Fs = np.eye(3)
+    Fs[0, 1] = g
+    s = O.stress(K, G, mat_to_kf(Fs))
+    assert abs(s[3] - G * g) <= 1e-11 * G * g
+
+
+def test_deformation_gradient_affine_exact():
+    rng = np.random.default_rng(8)
+    X, _ = random_elements(50, seed=9, amps=(0.0,))
+    for e in range(len(X)):
+        A = np.eye(3) + 0.05 * rng.normal(size=(3, 3))
+        c = rng.normal(size=3)
+        u = X[e] @ (A - np.eye(3)).T + c  # x = A X + c
+        F = O.deformation_gradients(X[e:e + 1], u[None])[0]
+        assert np.allclose(kf_to_mat(F), A[None], atol=1e-13)
+
+
+def test_forces_translation_invariance_and_patch():
+    X, u = random_elements(200, seed=10)
+    F = O.deformation_gradients(X, u)
+    s = O.stress(K, G, F)
+    f = O.nodal_forces(X, u, s)
+    scale = np.abs(f).max(axis=(1, 2))
+    assert np.all(np.abs(f.sum(axis=1)).max(axis=1) <= 1e-12 * scale)  # sum_a f_a = 0
+    # uniform stress on the unit cube: face load sigma_ij n_j * area/4 per node
+    sig = np.tile(np.array([3.0, -2.0, 5.0, 0.5, -0.25, 1.5]), (1, 8, 1))
+    f = O.nodal_forces(UNIT_HEX[None], np.zeros((1, 8, 3)), sig)[0]
+    S = sym_to_mat(sig[0, 0])
+    for a, Xa in enumerate(UNIT_HEX):
+        n = np.where(Xa > 0.5, 1.0, -1.0)  # outward normals of the 3 faces at node a
+        expect = -np.array([S[:, d] * n[d] for d in range(3)]).sum(axis=0) / 4.0
+        assert np.allclose(f[a], expect, atol=1e-14)
+
+
+def test_lumped_mass_total_and_regular_element():
+    m = O.lumped_mass_elem(7800.0, UNIT_HEX * 0.5)
+    assert np.allclose(m, 7800.0 * 0.125 / 8.0, rtol=1e-15)
+    X, _ = random_elements(30, seed=12, amps=(0.0,))
+    for e in range(len(X)):
+        vol = 0.0
+        # volume of a trilinear hex = sum over Gauss points of det J
+        F = O.deformation_gradients(X[e:e + 1], np.zeros((1, 8, 3)))
+        assert np.allclose(kf_to_mat(F[0]), np.eye(3), atol=1e-14)
+        m = O.lumped_mass_elem(1.0, X[e])
+        ip, N, dN = O.hex_tables()
+        for q in range(8):
+            J = X[e].T @ dN[q]
+            vol += np.linalg.det(J)
+        assert abs(m.sum() - vol) <= 1e-14 * vol
+
+
+def test_characteristic_length():
+    assert O.char_length(UNIT_HEX * 0.25) == 0.25
+    X = UNIT_HEX * np.array([1.0, 2.0, 3.0])
+    assert O.char_length(X) == 1.0
+
+
+def test_model_level_invariants():
+    X, conn = O.jittered_cube(4, 7, 0.1)
+    M = O.Model(X, conn, [0, len(conn)], [7800.0], [K], [G])
+    m = M.lumped_mass()
+    ip, N, dN = O.hex_tables()
+    assert abs(m.sum() - 7800.0) <= 1e-12 * 7800.0  # unit cube, total mass rho*V
+    # rigid translation: forces vanish; affine stretch: interior nodes balance
+    f = M.internal_force(np.tile([1e-3, -2e-3, 5e-4], (len(X), 1)))
+    assert np.max(np.abs(f)) < 1e-3
+    A = np.diag([1e-4, -5e-5, 2e-5])
+    f = M.internal_force(X @ A.T)
+    n = 4
+    interior = [i + (n + 1) * (j + (n + 1) * k) for i in range(1, n) for j in range(1, n) for k in range(1, n)]
+    assert np.max(np.abs(f[interior])) <= 1e-9 * np.max(np.abs(f))
+    assert np.abs(f.sum(axis=0)).max() <= 1e-9 * np.abs(f).max()
+
+
+def test_omp_path_is_bit_identical_to_serial():
+    X, conn = O.jittered_cube(6, 3, 0.1)
+    half = len(conn) // 2
+    M = O.Model(X, conn, [0, half, len(conn)], [7800.0, 2700.0], [K, 68.63e9], [G, 26.32e9],
+                np.arange(49, dtype=np.int32), np.full(49, 2, np.int32), np.ones(49, np.int32), np.full(49, 1.0))
+    runs = []
+    for threads in (1, 4):
+        r = O.ExplicitRun(M, 50 * 1e-7, 50, 10, threads=threads, reaction_nodes=np.arange(49), reaction_coord=2)
+        r.advance(50)
+        runs.append(r)
+    a, b = runs
+    assert np.array_equal(a.u, b.u) and np.array_equal(a.v, b.v) and np.array_equal(a.f, b.f)
+    assert np.array_equal(a.reaction, b.reaction)
+
+
+def test_planted_mistakes_are_caught():
+    """Each planted error in a copy of the stress fails the closed-form check."""
+    X, u = random_elements(20, seed=13, amps=(1e-3,))
+    F = O.deformation_gradients(X, u).reshape(-1, 9)[:40]
+    s = O.stress(K, G, F)
+    ref = np.array([closed_form_stress(kf_to_mat(Fp), K, G) for Fp in F])
+    good = max(np.linalg.norm(sym_to_mat(s[p]) - ref[p]) / np.linalg.norm(ref[p]) for p in range(len(F)))
+    assert good < 1e-11
+    mistakes = {
+        "sign of pressure": lambda S, Fp: S - 2 * np.trace(S) / 3 * np.eye(3),
+        "transposed F": lambda S, Fp: closed_form_stress(Fp.T, K, G),
+        "dropped J^-2/3": lambda S, Fp: S + G / np.linalg.det(Fp) * (1 - np.linalg.det(Fp) ** (-2 / 3))
+        * (Fp @ Fp.T - np.trace(Fp @ Fp.T) / 3 * np.eye(3)),
+        "K and G swapped": lambda S, Fp: closed_form_stress(Fp, G, K),
+    }
+    for name, plant in mistakes.items():
+        bad = max(np.linalg.norm(plant(sym_to_mat(s[p]), kf_to_mat(F[p])) - ref[p]) / np.linalg.norm(ref[p])
+                  for p in range(len(F)))
+        assert bad > 1e-9, name
