@@ -2,9 +2,8 @@
 //
 // These device functions reproduce NimbleSM's operation order exactly (the
 // file including this header is compiled with -ffp-contract=off), so that the
-// GPU result equals the reference's FP64 result bit for bit wherever the
-// reference's libm cbrt is correctly rounded.  Citations are NimbleSM
-// file:line.
+// GPU result equals the reference's FP64 result bit for bit.  Citations are
+// NimbleSM file:line.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -66,35 +65,34 @@ __device__ __forceinline__ double invert_full33(const double* a, double* inv)
   return det;
 }
 
-// Correctly rounded cube root (round to nearest, ties impossible for a
-// non-exact cube root).  Range reduction a = m 2^{3q}, m in [0.5, 4); a
-// single-precision log2/exp2 start (~2^-22), two Newton steps in double,
-// then one correction y += r / (3 y^2) with the residual r = m - y^3
-// evaluated to ~2^-106 through FMA-exact products (Dekker's TwoProduct).
-// The reference links glibc's cbrt, which is correctly rounded for all but a
-// few in 10^4 arguments near 1, so this keeps the stress bit-identical to the
-// reference's in all but those rare cases (the device library's cbrt is
-// not correctly rounded).
-__device__ __forceinline__ double cbrt_rn(double x)
+// Cube root by the method of the platform libm the reference links (GNU libc
+// dbl-64 cbrt, Alboth & Drepper 1997): reduce |x| = xm 2^xe with xm in
+// [0.5, 1), a degree-6 polynomial start u(xm), one rational correction
+// u (u^3 + 2 xm) / (2 u^3 + xm), a 2^{(xe mod 3)/3} factor and an exact
+// scaling by 2^{xe/3}.  That libm cbrt is not correctly rounded (about 45% of
+// arguments near 1 are 1-3 ulp off), and NimbleSM's stress calls it
+// (src/nimble_material.cc:274), so the parity path evaluates the same steps
+// with the same constants to reproduce the reference's FP64 bits.
+__device__ __forceinline__ double cbrt_ref(double x)
 {
-  if (!(fabs(x) > 0.0) || isinf(x)) return x + x;  // 0, -0, inf, nan
-  const double a = fabs(x);
-  int          e;
-  double       m = frexp(a, &e);  // a = m 2^e, m in [0.5, 1)
-  int          q = e >= 0 ? e / 3 : -((-e + 2) / 3);
-  m            = ldexp(m, e - 3 * q);  // m in [0.5, 4)
-  double y     = (double)__builtin_amdgcn_exp2f(__builtin_amdgcn_logf((float)m) * (1.0f / 3.0f));
-  y            = y - (y * y * y - m) / (3.0 * y * y);
-  y            = y - (y * y * y - m) / (3.0 * y * y);
-  // r = m - y^3 to double-double accuracy
-  const double y2h = y * y;
-  const double y2l = fma(y, y, -y2h);
-  const double y3h = y * y2h;
-  const double y3l = fma(y, y2h, -y3h) + y * y2l;
-  const double r   = (m - y3h) - y3l;
-  y                = y + r / (3.0 * y2h);
-  y                = ldexp(y, q);
-  return x < 0.0 ? -y : y;
+  if (x == 0.0 || isnan(x) || isinf(x)) return x + x;
+  int          xe;
+  const double xm = frexp(fabs(x), &xe);
+  const double u  = (0.354895765043919860 +
+                    ((1.50819193781584896 +
+                      ((-2.11499494167371287 +
+                        ((2.44693122563534430 +
+                          ((-1.83469277483613086 + (0.784932344976639262 - 0.145263899385486377 * xm) * xm) * xm)) *
+                         xm)) *
+                       xm)) *
+                     xm));
+  const double t2  = u * u * u;
+  const double cb2 = 1.2599210498948731648;   // 2^(1/3)
+  const double sq2 = 1.5874010519681994748;   // 2^(2/3)
+  const int    r   = xe % 3;                  // C remainder: -2..2
+  const double fac = r == -2 ? 1.0 / sq2 : r == -1 ? 1.0 / cb2 : r == 0 ? 1.0 : r == 1 ? cb2 : sq2;
+  const double ym  = u * (t2 + 2.0 * xm) / (2.0 * t2 + xm) * fac;
+  return ldexp(x > 0.0 ? ym : -ym, xe / 3);
 }
 
 // x times the sign of y (src/nimble_utils.h:131-136)
@@ -287,7 +285,7 @@ __device__ __forceinline__ void neohookean_stress(double K, double G, const doub
   const double xj = v[S_XX] * v[S_YY] * v[S_ZZ] + 2.0 * v[S_XY] * v[S_YZ] * v[S_ZX] - v[S_XX] * v[S_YZ] * v[S_YZ] -
                     v[S_YY] * v[S_ZX] * v[S_ZX] - v[S_ZZ] * v[S_XY] * v[S_XY];
 
-  const double cj       = cbrt_rn(xj);
+  const double cj       = cbrt_ref(xj);
   const double fac      = 1.0 / (cj * cj);
   const double pressure = 0.5 * K * (xj - 1.0 / xj);
 

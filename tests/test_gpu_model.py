@@ -1,8 +1,8 @@
 """Device ModelData + explicit driver against the oracle.
 
-FP64: nodal forces, displacements, velocities, F, stress and the reaction
-history match the oracle to 1e-12 relative (the reference's FP64 run on the
-same input).  Mixed precision (FP32 element math, FP64 node state): errors
+FP64: nodal forces, displacements, velocities, F and stress are bit-identical
+to the oracle (the reference's FP64 run on the same input); the reaction
+history, a parallel tree sum on the device, agrees to 1e-14 relative.  Mixed precision (FP32 element math, FP64 node state): errors
 against the FP64 oracle are bounded per output as stated in each test.
 """
 import numpy as np
@@ -82,8 +82,9 @@ def test_reference_decks_fp64(hip_device, name):
         scale = max(np.abs(r).max() for r in refs[k])
         err = max(np.abs(g - r).max() for g, r in zip(gots[k], refs[k])) / scale
         print(f"{name} {k}: {err:.2e}")
-        bound = {"F": 1e-13, "stress": 1e-11}.get(k, 1e-12)  # target: u, v, a, f to 1e-12
-        assert err <= bound, (name, k, err)
+        # the FP64 path keeps the reference's operation order and libm cube-root
+        # method: every field is bit-identical to the oracle
+        assert err == 0.0, (name, k, err)
 
 
 def test_cube16_fp64_1000_steps(hip_device):
@@ -95,9 +96,10 @@ def test_cube16_fp64_1000_steps(hip_device):
     assert integ.time_current == run.time
     for label, ref in (("displacement", run.u), ("velocity", run.v), ("internal_force", run.f)):
         got = m.GetVectorNodeData(label)
-        assert rel_err(got, ref) <= 1e-12 and max_rel_err(got, ref) <= 1e-12, (label, rel_err(got, ref))
+        assert np.array_equal(got, ref), (label, rel_err(got, ref))  # bit-identical (target: 1e-12)
     R = m.GetReactionHistory(0, steps)
-    assert max_rel_err(R, run.reaction) <= 1e-12
+    # the reaction is a parallel tree sum on the device (a serial sum in the oracle)
+    assert max_rel_err(R, run.reaction) <= 1e-14
     assert np.abs(R).max() > 0
 
 
@@ -112,7 +114,7 @@ def test_one_step_internal_force_two_blocks(hip_device):
     f, Fip, sip = om.internal_force(u, want_ip=True)
     assert np.array_equal(m.GetVectorNodeData("internal_force"), f)
     assert np.array_equal(m.GetIntegrationPointData("deformation_gradient"), Fip)
-    assert max_rel_err(m.GetIntegrationPointData("stress"), sip) <= 1e-14
+    assert np.array_equal(m.GetIntegrationPointData("stress"), sip)
 
 
 def test_small_and_empty_blocks(hip_device):
@@ -122,7 +124,7 @@ def test_small_and_empty_blocks(hip_device):
                                num_steps=20)):
         m, integ = run_gpu_problem(prob, prob.num_steps, "fp64")
         run, _ = run_oracle_problem(prob, prob.num_steps, threads=1)
-        assert max_rel_err(m.GetVectorNodeData("displacement"), run.u) <= 1e-12
+        assert np.array_equal(m.GetVectorNodeData("displacement"), run.u)
     X, conn = N.jittered_cube(2, 3, 0.1)
     blocks = [N.Block(1, conn[:4], N.NeohookeanMaterial(7800, 1e11, 5e10)),
               N.Block(2, conn[:0], N.NeohookeanMaterial(2700, 7e10, 3e10)),

@@ -190,3 +190,44 @@ def test_planted_mistakes_are_caught():
         bad = max(np.linalg.norm(plant(sym_to_mat(s[p]), kf_to_mat(F[p])) - ref[p]) / np.linalg.norm(ref[p])
                   for p in range(len(F)))
         assert bad > 1e-9, name
+
+
+def _cbrt_libm_method(x):
+    """Restatement of the cube-root method the FP64 device path uses (the
+    platform libm's: polynomial start, one rational correction, 2^{k/3}
+    factor, exact scaling) in float64 scalar arithmetic."""
+    import math
+
+    F = np.float64
+    xm, xe = math.frexp(abs(float(x)))
+    xm = F(xm)
+    if x == 0 or not np.isfinite(x):
+        return F(x) + F(x)
+    u = (F(0.354895765043919860) + ((F(1.50819193781584896) + ((F(-2.11499494167371287) + ((F(2.44693122563534430) + (
+        (F(-1.83469277483613086) + (F(0.784932344976639262) - F(0.145263899385486377) * xm) * xm) * xm)) * xm))
+        * xm)) * xm))
+    t2 = u * u * u
+    cb2, sq2 = F(1.2599210498948731648), F(1.5874010519681994748)
+    fac = [F(1.0) / sq2, F(1.0) / cb2, F(1.0), cb2, sq2][2 + int(math.fmod(xe, 3))]
+    ym = u * (t2 + F(2.0) * xm) / (F(2.0) * t2 + xm) * fac
+    q = int(xe / 3) if xe >= 0 else -int(-xe / 3)
+    return F(math.ldexp(float(ym if x > 0 else -ym), q))
+
+
+def test_cbrt_method_matches_platform_libm():
+    """The stress calls std::cbrt (src/nimble_material.cc:274); the device FP64
+    path reproduces the x86-64 GNU libm method bit for bit.  That libm result is
+    not correctly rounded, so a correctly rounded cube root would not do."""
+    import ctypes
+
+    libm = ctypes.CDLL("libm.so.6")
+    libm.cbrt.restype = ctypes.c_double
+    libm.cbrt.argtypes = [ctypes.c_double]
+    rng = np.random.default_rng(17)
+    xs = np.concatenate([1 + rng.uniform(-1e-3, 1e-3, 20000), 1 + rng.uniform(-1e-9, 1e-9, 5000),
+                         np.exp(rng.uniform(-600, 600, 10000)), -rng.uniform(0.1, 10, 5000), [1.0, 8.0, 0.125]])
+    got = np.array([_cbrt_libm_method(x) for x in xs])
+    ref = np.array([libm.cbrt(float(x)) for x in xs])
+    assert np.array_equal(got, ref)
+    # and the method is not the correctly rounded cube root
+    assert np.sum(got[:20000] != np.cbrt(xs[:20000])) > 1000
