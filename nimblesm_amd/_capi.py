@@ -7,7 +7,9 @@ import ctypes as C
 import os
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libnimblesm_amd.so")
+# NSM_LIBRARY selects another build of the library (tuning builds); the
+# in-tree build is the default
+LIB_PATH = os.environ.get("NSM_LIBRARY") or os.path.join(HERE, "libnimblesm_amd.so")
 
 NSM_OK, NSM_ERR_ARG, NSM_ERR_HIP, NSM_ERR_NEG_JACOBIAN, NSM_ERR_STATE, NSM_ERR_NOMEM = range(6)
 NSM_FP64, NSM_MIXED = 0, 1

@@ -45,20 +45,22 @@ def _run(cmd):
             sys.stderr.write(line + "\n")
 
 
-def build(verbose: bool = False) -> str:
-    os.makedirs(OBJ, exist_ok=True)
+def build(verbose: bool = False, out: str = LIB, extra_flags=(), obj_dir: str = OBJ) -> str:
+    """Compile every translation unit and link `out` (the in-tree library by
+    default; `extra_flags`, e.g. -DNSM_MIXED_MIN_WAVES=3, for tuning builds)."""
+    os.makedirs(obj_dir, exist_ok=True)
     objs = []
     for src, flags in SOURCES.items():
-        obj = os.path.join(OBJ, src + ".o")
-        cmd = [HIPCC, *COMMON, *flags, "-c", "-o", obj, os.path.join(CSRC, src)]
+        obj = os.path.join(obj_dir, src + ".o")
+        cmd = [HIPCC, *COMMON, *flags, *extra_flags, "-c", "-o", obj, os.path.join(CSRC, src)]
         if verbose:
             print(" ".join(cmd))
         _run(cmd)
         objs.append(obj)
-    tmp = LIB + ".tmp"
+    tmp = out + ".tmp"
     _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs])
-    os.replace(tmp, LIB)
-    return LIB
+    os.replace(tmp, out)
+    return out
 
 
 if __name__ == "__main__":

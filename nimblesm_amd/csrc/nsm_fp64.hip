@@ -3,8 +3,8 @@
 // seam kernels and the lumped-mass / critical-time-step setup kernels.
 //
 // Compiled with -ffp-contract=off: every FP64 expression keeps NimbleSM's
-// evaluation order so results match the reference's FP64 run bit for bit
-// (cbrt aside).  Work mapping: one 64-wide wavefront holds 8 elements, lane
+// evaluation order so results match the reference's FP64 run bit for bit.
+// Work mapping: one 64-wide wavefront holds 8 elements, lane
 // k of an element serves as node k (gather, force output) and as Gauss point
 // k (F, stress); LDS carries the per-element node data and per-point
 // inverse Jacobians between the two roles.
@@ -171,7 +171,7 @@ struct Lds {
   };
 };
 
-__global__ __launch_bounds__(kThreads) void k_element_fp64(int64_t e_begin, int64_t e_end,
+__global__ __launch_bounds__(kThreads) void k_element_fp64(int64_t e_begin, int64_t e_end, int64_t E_total,
                                                            const int* __restrict__ conn,
                                                            const double* __restrict__ X,
                                                            const double* __restrict__ u, double K, double G,
@@ -181,7 +181,7 @@ __global__ __launch_bounds__(kThreads) void k_element_fp64(int64_t e_begin, int6
   __shared__ Lds lds;
   const int     el    = threadIdx.x >> 3;
   const int     k     = threadIdx.x & 7;
-  const int64_t e     = e_begin + (int64_t)blockIdx.x * kEPW + el;
+  const int64_t e     = e_begin + (int64_t)xcd_tile(blockIdx.x, gridDim.x) * kEPW + el;
   const bool    valid = e < e_end;
 
   if (valid) {
@@ -202,13 +202,12 @@ __global__ __launch_bounds__(kThreads) void k_element_fp64(int64_t e_begin, int6
     jacobians(k, lds.nodes.x[el], lds.nodes.u[el], a, b);
     bad |= invert3x3(b, bi) <= 0.0;
     def_grad(a, bi, F);
-    double* Fo = Fip + (e * 8 + k) * 9;
+    // element-fastest [comp][ipt][E] layout
 #pragma unroll
-    for (int i = 0; i < 9; i++) Fo[i] = F[i];
+    for (int i = 0; i < 9; i++) Fip[(int64_t)(i * 8 + k) * E_total + e] = F[i];
     neohookean_stress(K, G, F, sig);
-    double* so = sip + (e * 8 + k) * 6;
 #pragma unroll
-    for (int i = 0; i < 6; i++) so[i] = sig[i];
+    for (int i = 0; i < 6; i++) sip[(int64_t)(i * 8 + k) * E_total + e] = sig[i];
     // the force routine rebuilds a from the same x = X + u in the same order
     ainv[9] = invert3x3(a, ainv);
     bad |= ainv[9] <= 0.0;
@@ -224,10 +223,9 @@ __global__ __launch_bounds__(kThreads) void k_element_fp64(int64_t e_begin, int6
   if (valid) {
     double f[3];
     node_force(k, lds.pts.ai[el], lds.pts.sg[el], f);
-    double* so = slots + (e * 8 + k) * 3;
-    so[0]      = f[0];
-    so[1]      = f[1];
-    so[2]      = f[2];
+    slots[(int64_t)k * E_total + e]       = f[0];  // [comp][node][E]
+    slots[(int64_t)(8 + k) * E_total + e]  = f[1];
+    slots[(int64_t)(16 + k) * E_total + e] = f[2];
   }
   if (bad) atomicOr(status, 1);
 }
@@ -379,7 +377,7 @@ __global__ __launch_bounds__(256) void k_seam_char_length(int64_t E, const doubl
 // ---------------------------------------------------------------------------
 // Model setup
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(kThreads) void k_element_mass(int64_t e_begin, int64_t e_end,
+__global__ __launch_bounds__(kThreads) void k_element_mass(int64_t e_begin, int64_t e_end, int64_t E_total,
                                                            const int* __restrict__ conn,
                                                            const double* __restrict__ X, double rho,
                                                            double* __restrict__ elem_mass, int* __restrict__ status)
@@ -402,7 +400,7 @@ __global__ __launch_bounds__(kThreads) void k_element_mass(int64_t e_begin, int6
     if (det[el][k] <= 0.0) atomicOr(status, 1);
   }
   __syncthreads();
-  if (e < e_end) lumped_row(k, rho, det[el], &elem_mass[e * 8 + k]);
+  if (e < e_end) lumped_row(k, rho, det[el], &elem_mass[(int64_t)k * E_total + e]);
 }
 
 // Minimum over the workgroup's elements of charlen / sound_speed on current
@@ -444,12 +442,13 @@ inline unsigned grid_of(int64_t n, int64_t per) { return (unsigned)((n + per - 1
 
 }  // namespace
 
-void launch_element_fp64(int64_t e_begin, int64_t e_end, const int* conn, const double* X, const double* u, double K,
-                         double G, double* Fip, double* sip, double* slots, int* status, hipStream_t s)
+void launch_element_fp64(int64_t e_begin, int64_t e_end, int64_t E_total, const int* conn, const double* X,
+                         const double* u, double K, double G, double* Fip, double* sip, double* slots, int* status,
+                         hipStream_t s)
 {
   if (e_end <= e_begin) return;
   hipLaunchKernelGGL(k_element_fp64, dim3(grid_of(e_end - e_begin, kEPW)), dim3(kThreads), 0, s, e_begin, e_end,
-                     conn, X, u, K, G, Fip, sip, slots, status);
+                     E_total, conn, X, u, K, G, Fip, sip, slots, status);
 }
 
 void launch_seam_defgrad(int64_t E, const double* Xg, const double* ug, double* F, int* status, hipStream_t s)
@@ -483,12 +482,12 @@ void launch_seam_char_length(int64_t E, const double* xg, double* out, hipStream
   hipLaunchKernelGGL(k_seam_char_length, dim3(grid_of(E, 256)), dim3(256), 0, s, E, xg, out);
 }
 
-void launch_element_mass(int64_t e_begin, int64_t e_end, const int* conn, const double* X, double rho,
-                         double* elem_mass, int* status, hipStream_t s)
+void launch_element_mass(int64_t e_begin, int64_t e_end, int64_t E_total, const int* conn, const double* X,
+                         double rho, double* elem_mass, int* status, hipStream_t s)
 {
   if (e_end <= e_begin) return;
   hipLaunchKernelGGL(k_element_mass, dim3(grid_of(e_end - e_begin, kEPW)), dim3(kThreads), 0, s, e_begin, e_end,
-                     conn, X, rho, elem_mass, status);
+                     E_total, conn, X, rho, elem_mass, status);
 }
 
 int launch_element_dt(int64_t e_begin, int64_t e_end, const int* conn, const double* X, const double* u,

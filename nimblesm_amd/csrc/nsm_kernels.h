@@ -1,4 +1,12 @@
 // nsm_kernels.h -- host-side launchers of the HIP kernels (internal).
+//
+// Device layouts:
+//   node vectors          [N][3] double (the reference's AoS node layout)
+//   integration points    element-fastest "LayoutLeft": F [9][8][E],
+//                         sigma [6][8][E] (component, Gauss point, element)
+//   element force slots   [3][8][E] (component, local node, element)
+//   node -> element CSR   node_ent[p] = k*E + e for the p-th incidence
+//                         (element e, local node k), ascending e per node
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -6,13 +14,18 @@
 
 namespace nsm {
 
-// Per-(node, coordinate) resolved kinematic condition.  kind: 0 none,
-// 1 prescribed velocity, 2 prescribed displacement.
+// Per-node resolved kinematic condition.  kind: 0 none, 1 prescribed
+// velocity, 2 prescribed displacement.
 struct BcEntry {
   int32_t kind[3];
   int32_t pad;
   double  value[3];
 };
+
+// node_flags: low 7 bits = bc_table index + 1 (0: free), bit 7 = node belongs
+// to the reaction-force set
+constexpr uint8_t kReactionFlag = 0x80;
+constexpr int     kMaxBcEntries = 127;
 
 // Times of one explicit step, computed on the host exactly as
 // src/integrators/explicit_time_integrator.cc:192-195 does.
@@ -22,22 +35,41 @@ struct StepTimes {
 
 struct NodeArrays {
   int64_t         num_nodes;
-  const uint32_t* node_ptr;  // [N+1] CSR offsets into node_ent
-  const uint32_t* node_ent;  // 8*element + local node, ascending element
-  const double*   mass;      // [N]
-  double*         u;         // [N][3]
-  double*         v;         // [N][3]
-  double*         a;         // [N][3]
-  double*         f;         // [N][3] internal force
-  const uint8_t*  bc_code;   // [N], 0 = free, else index+1 into bc_table
+  int64_t         slot_stride;  // 8*E: distance between force components in the slots
+  const uint32_t* node_ptr;     // [N+1] CSR offsets into node_ent
+  const uint32_t* node_ent;
+  const double*   mass;         // [N]
+  double*         u;            // [N][3]
+  double*         v;            // [N][3]
+  double*         a;            // [N][3]
+  double*         f;            // [N][3] internal force
+  const uint8_t*  flags;        // [N]
   const BcEntry*  bc_table;
 };
 
+// Reaction-force sum: the node kernel writes one partial per node tile
+// (256 nodes) that holds reaction nodes; launch_reaction_finalize then sums
+// the partials of those tiles in ascending tile order.
+struct ReactionArgs {
+  int     coord;     // -1: no reaction sum
+  double* partials;  // [node tiles]
+};
+
+// Workgroup b of nb -> tile index so that XCD (b mod 8) processes one
+// contiguous range of tiles (workgroups are dealt to the 8 XCDs round-robin).
+__device__ __forceinline__ unsigned xcd_tile(unsigned b, unsigned nb)
+{
+  const unsigned x = b & 7u, k = b >> 3, q = nb >> 3, r = nb & 7u;
+  return x * q + (x < r ? x : r) + k;
+}
+
 // ---- element kernels (element range [e_begin, e_end) of one block) ----
-void launch_element_fp64(int64_t e_begin, int64_t e_end, const int* conn, const double* X, const double* u, double K,
-                         double G, double* Fip, double* sip, double* slots, int* status, hipStream_t s);
-void launch_element_mixed(int64_t e_begin, int64_t e_end, const int* conn, const double* X, const double* u, float K,
-                          float G, float* Fip, float* sip, float* slots, int* status, hipStream_t s);
+void launch_element_fp64(int64_t e_begin, int64_t e_end, int64_t E_total, const int* conn, const double* X,
+                         const double* u, double K, double G, double* Fip, double* sip, double* slots, int* status,
+                         hipStream_t s);
+void launch_element_mixed(int64_t e_begin, int64_t e_end, int64_t E_total, const int* conn, const double* X,
+                          const double* u, float K, float G, float* Fip, float* sip, float* slots, int* status,
+                          hipStream_t s);
 
 // ---- element / material seams on gathered arrays ----
 void launch_seam_defgrad(int64_t E, const double* Xg, const double* ug, double* F, int* status, hipStream_t s);
@@ -48,8 +80,9 @@ void launch_seam_lumped_mass(int64_t E, double rho, const double* Xg, double* m,
 void launch_seam_char_length(int64_t E, const double* xg, double* out, hipStream_t s);
 
 // ---- model setup ----
-void launch_element_mass(int64_t e_begin, int64_t e_end, const int* conn, const double* X, double rho,
-                         double* elem_mass, int* status, hipStream_t s);
+// element masses in slot layout: elem_mass[k*E_total + e]
+void launch_element_mass(int64_t e_begin, int64_t e_end, int64_t E_total, const int* conn, const double* X,
+                         double rho, double* elem_mass, int* status, hipStream_t s);
 // per-workgroup minimum of charlen/c over [e_begin, e_end) into wg_min[blockIdx]
 int  launch_element_dt(int64_t e_begin, int64_t e_end, const int* conn, const double* X, const double* u,
                        double sound_speed, double* wg_min, hipStream_t s);
@@ -58,19 +91,18 @@ void launch_mass_gather(int64_t N, const uint32_t* node_ptr, const uint32_t* nod
                         double* m, hipStream_t s);
 
 // ---- node kernels ----
-// Kinematic conditions on v (and nothing else): the explicit scheme's
-// ApplyKinematicBC.
+unsigned node_grid(int64_t num_nodes);
+// Kinematic conditions on v: the explicit scheme's ApplyKinematicBC.
 void launch_apply_bc(const NodeArrays& na, StepTimes t, hipStream_t s);
 // first half kick + BC + drift + BC of one step
 void launch_prekick(const NodeArrays& na, StepTimes t, hipStream_t s);
 // gather f from element slots, a = f/m, second half kick (+BC on output
-// steps); if fuse_next, also the next step's first half kick, BC, drift, BC.
+// steps); if fuse_next, also the next step's first half kick, BC, drift, BC;
+// optional reaction sum.
 template <typename SlotT>
 void launch_node_update(const NodeArrays& na, const SlotT* slots, StepTimes cur, bool output_bc, bool fuse_next,
-                        StepTimes next, bool write_a, bool update_v, hipStream_t s);
-// gather only (ComputeInternalForce without the integrator)
-template <typename SlotT>
-void launch_force_gather(const NodeArrays& na, const SlotT* slots, hipStream_t s);
-void launch_reaction(const int* nodes, int64_t n, int coord, const double* f, double* out, hipStream_t s);
+                        StepTimes next, bool write_a, bool update_v, ReactionArgs rx, hipStream_t s);
+// out = -sum_k partials[tiles[k]], k ascending (fixed tree, reproducible)
+void launch_reaction_finalize(const double* partials, const int* tiles, int num_tiles, double* out, hipStream_t s);
 
 }  // namespace nsm

@@ -80,6 +80,13 @@ int dalloc(T** p, size_t count, size_t* total)
   return NSM_OK;
 }
 
+template <typename T>
+void free_dev(T*& p)
+{
+  if (p) (void)hipFree((void*)p);
+  p = nullptr;
+}
+
 struct BcRecord {
   int              kind, coord;
   double           value;
@@ -113,23 +120,26 @@ struct nsm_model {
   int*      conn = nullptr;
   uint32_t* node_ptr = nullptr;
   uint32_t* node_ent = nullptr;
-  void*     Fip = nullptr;    // double or float [E][8][9]
-  void*     sip = nullptr;    // double or float [E][8][6]
-  void*     slots = nullptr;  // double or float [E][8][3]
-  double*   elem_tmp = nullptr;  // [8E] setup scratch (element masses)
+  void*     Fip = nullptr;       // double or float [9][8][E] (component, ipt, element)
+  void*     sip = nullptr;       // double or float [6][8][E]
+  void*     slots = nullptr;     // double or float [3][8][E] element force slots
+  double*   elem_tmp = nullptr;  // [8][E] setup scratch (element masses)
   double*   wg_min = nullptr;
   int64_t   wg_min_len = 0;
   int*      status = nullptr;
-  uint8_t*  bc_code = nullptr;
+  uint8_t*  flags = nullptr;  // [N] bc table index + 1 | reaction bit
   nsm::BcEntry* bc_table = nullptr;
-  int*      reaction_nodes = nullptr;
   int64_t   num_reaction_nodes = 0;
   int       reaction_coord = 2;
   double*   reaction_hist = nullptr;
   int64_t   reaction_cap = 0;
+  double*   reaction_partials = nullptr;  // [node tiles]
+  int*      reaction_tiles = nullptr;     // node tiles holding reaction nodes, ascending
+  int       num_reaction_tiles = 0;
   size_t    device_bytes = 0;
 
   std::vector<BcRecord> bcs;
+  std::vector<uint8_t>  reaction_mask;  // host copy of the reaction bit per node
   bool                  bc_dirty = true;
   int                   num_bc_entries = 0;
   bool                  have_mass = false;
@@ -143,15 +153,16 @@ struct nsm_model {
   nsm::NodeArrays node_arrays() const
   {
     nsm::NodeArrays na;
-    na.num_nodes = N;
-    na.node_ptr  = node_ptr;
+    na.num_nodes   = N;
+    na.slot_stride = 8 * E;
+    na.node_ptr    = node_ptr;
     na.node_ent  = node_ent;
     na.mass      = m;
     na.u         = u;
     na.v         = v;
     na.a         = a;
     na.f         = f;
-    na.bc_code   = bc_code;
+    na.flags     = flags;
     na.bc_table  = bc_table;
     return na;
   }
@@ -224,8 +235,9 @@ int rebuild_bcs(nsm_model* md)
         break;
       }
     if (idx < 0) {
-      if (table.size() >= 255)
-        return fail(NSM_ERR_ARG, "more than 255 distinct kinematic-condition combinations are not supported");
+      if ((int)table.size() >= nsm::kMaxBcEntries)
+        return fail(NSM_ERR_ARG, "more than %d distinct kinematic-condition combinations are not supported",
+                    nsm::kMaxBcEntries);
       idx = (int)table.size();
       table.push_back(e);
     }
@@ -240,7 +252,23 @@ int rebuild_bcs(nsm_model* md)
   if (st) return st;
   if (!table.empty())
     NSM_HIP(hipMemcpy(md->bc_table, table.data(), table.size() * sizeof(nsm::BcEntry), hipMemcpyHostToDevice));
-  NSM_HIP(hipMemcpy(md->bc_code, code.data(), code.size(), hipMemcpyHostToDevice));
+  std::vector<int> tiles;
+  for (size_t i = 0; i < code.size(); i++)
+    if (md->reaction_mask.size() == code.size() && md->reaction_mask[i]) {
+      code[i] |= nsm::kReactionFlag;
+      const int t = (int)(i / 256);  // node tile of the node kernel
+      if (tiles.empty() || tiles.back() != t) tiles.push_back(t);
+    }
+  NSM_HIP(hipMemcpy(md->flags, code.data(), code.size(), hipMemcpyHostToDevice));
+  free_dev(md->reaction_tiles);
+  md->num_reaction_tiles = 0;
+  if (!tiles.empty()) {
+    size_t dummy2 = 0;
+    int    st2    = dalloc(&md->reaction_tiles, tiles.size(), &dummy2);
+    if (st2) return st2;
+    NSM_HIP(hipMemcpy(md->reaction_tiles, tiles.data(), tiles.size() * sizeof(int), hipMemcpyHostToDevice));
+    md->num_reaction_tiles = (int)tiles.size();
+  }
   md->num_bc_entries = (int)table.size();
   md->bc_dirty       = false;
   return NSM_OK;
@@ -266,12 +294,13 @@ int launch_elements(nsm_model* md)
     int    st = time_begin(md, NSM_KERNEL_ELEMENT, &ti);
     if (st) return st;
     if (md->precision == NSM_FP64)
-      nsm::launch_element_fp64(md->blk_off[b], md->blk_off[b + 1], md->conn, md->X, md->u, md->K[b], md->G[b],
-                               (double*)md->Fip, (double*)md->sip, (double*)md->slots, md->status, md->stream);
+      nsm::launch_element_fp64(md->blk_off[b], md->blk_off[b + 1], md->E, md->conn, md->X, md->u, md->K[b],
+                               md->G[b], (double*)md->Fip, (double*)md->sip, (double*)md->slots, md->status,
+                               md->stream);
     else
-      nsm::launch_element_mixed(md->blk_off[b], md->blk_off[b + 1], md->conn, md->X, md->u, (float)md->K[b],
-                                (float)md->G[b], (float*)md->Fip, (float*)md->sip, (float*)md->slots, md->status,
-                                md->stream);
+      nsm::launch_element_mixed(md->blk_off[b], md->blk_off[b + 1], md->E, md->conn, md->X, md->u,
+                                (float)md->K[b], (float)md->G[b], (float*)md->Fip, (float*)md->sip,
+                                (float*)md->slots, md->status, md->stream);
     NSM_LAUNCHED();
     st = time_end(md, ti);
     if (st) return st;
@@ -279,31 +308,30 @@ int launch_elements(nsm_model* md)
   return NSM_OK;
 }
 
+// Node kernel; the reaction sum of `step` (if a reaction set exists and the
+// step is recorded) is folded into it.
 int launch_node(nsm_model* md, nsm::StepTimes cur, bool output_bc, bool fuse, nsm::StepTimes nxt, bool write_a,
-                bool update_v)
+                bool update_v, int64_t step)
 {
   size_t ti;
   int    st = time_begin(md, NSM_KERNEL_NODE, &ti);
   if (st) return st;
+  nsm::ReactionArgs rx{-1, nullptr};
+  const bool        want_rx = md->num_reaction_tiles > 0 && step >= 0 && step < md->reaction_cap;
+  if (want_rx) rx = nsm::ReactionArgs{md->reaction_coord, md->reaction_partials};
   const nsm::NodeArrays na = md->node_arrays();
   if (md->precision == NSM_FP64)
-    nsm::launch_node_update<double>(na, (const double*)md->slots, cur, output_bc, fuse, nxt, write_a, update_v,
+    nsm::launch_node_update<double>(na, (const double*)md->slots, cur, output_bc, fuse, nxt, write_a, update_v, rx,
                                     md->stream);
   else
-    nsm::launch_node_update<float>(na, (const float*)md->slots, cur, output_bc, fuse, nxt, write_a, update_v,
+    nsm::launch_node_update<float>(na, (const float*)md->slots, cur, output_bc, fuse, nxt, write_a, update_v, rx,
                                    md->stream);
   NSM_LAUNCHED();
-  return time_end(md, ti);
-}
-
-int launch_reaction_at(nsm_model* md, int64_t step)
-{
-  if (md->num_reaction_nodes <= 0 || step < 0 || step >= md->reaction_cap) return NSM_OK;
-  size_t ti;
-  int    st = time_begin(md, NSM_KERNEL_REACTION, &ti);
-  if (st) return st;
-  nsm::launch_reaction(md->reaction_nodes, md->num_reaction_nodes, md->reaction_coord, md->f,
-                       md->reaction_hist + step, md->stream);
+  if ((st = time_end(md, ti))) return st;
+  if (!want_rx) return NSM_OK;
+  if ((st = time_begin(md, NSM_KERNEL_REACTION, &ti))) return st;
+  nsm::launch_reaction_finalize(md->reaction_partials, md->reaction_tiles, md->num_reaction_tiles,
+                                md->reaction_hist + step, md->stream);
   NSM_LAUNCHED();
   return time_end(md, ti);
 }
@@ -324,13 +352,6 @@ bool is_output(int64_t step, int64_t num_load_steps, int64_t output_frequency)
 {
   if (output_frequency == 0) return false;
   return step % output_frequency == 0 || step == num_load_steps - 1;
-}
-
-template <typename T>
-void free_dev(T*& p)
-{
-  if (p) (void)hipFree((void*)p);
-  p = nullptr;
 }
 
 }  // namespace
@@ -450,13 +471,14 @@ int nsm_model_create(int64_t num_nodes, const double* reference_coordinates, int
     if (conn[k] < 0 || conn[k] >= num_nodes)
       return fail(NSM_ERR_ARG, "connectivity entry %lld = %d out of range", (long long)k, conn[k]);
 
-  // node -> (8*element + local node) incidence in ascending element order
+  // node -> (element e, local node j) incidence in ascending element order,
+  // stored as the slot offset j*E + e of the element-fastest force slots
   std::vector<uint32_t> ptr((size_t)num_nodes + 1, 0), ent((size_t)(8 * E));
   for (int64_t k = 0; k < 8 * E; k++) ptr[(size_t)conn[k] + 1]++;
   for (int64_t i = 0; i < num_nodes; i++) ptr[(size_t)i + 1] += ptr[(size_t)i];
   {
     std::vector<uint32_t> fill(ptr.begin(), ptr.end() - 1);
-    for (int64_t k = 0; k < 8 * E; k++) ent[fill[(size_t)conn[k]]++] = (uint32_t)k;
+    for (int64_t k = 0; k < 8 * E; k++) ent[fill[(size_t)conn[k]]++] = (uint32_t)((k & 7) * E + (k >> 3));
   }
 
   nsm_model* md  = new nsm_model();
@@ -489,7 +511,8 @@ int nsm_model_create(int64_t num_nodes, const double* reference_coordinates, int
       (st = dalloc((char**)&md->Fip, (size_t)(72 * E) * ipw, &tot)) ||
       (st = dalloc((char**)&md->sip, (size_t)(48 * E) * ipw, &tot)) ||
       (st = dalloc((char**)&md->slots, (size_t)(24 * E) * ipw, &tot)) ||
-      (st = dalloc(&md->status, 1, &tot)) || (st = dalloc(&md->bc_code, (size_t)num_nodes, &tot)))
+      (st = dalloc(&md->status, 1, &tot)) || (st = dalloc(&md->flags, (size_t)num_nodes, &tot)) ||
+      (st = dalloc(&md->reaction_partials, (size_t)nsm::node_grid(num_nodes), &tot)))
     return cleanup(st);
 
   hipStream_t s = md->stream;
@@ -510,19 +533,18 @@ int nsm_model_create(int64_t num_nodes, const double* reference_coordinates, int
   NSM_HIP_C(hipMemsetAsync(md->f, 0, N3 * sizeof(double), s));
   NSM_HIP_C(hipMemsetAsync(md->m, 0, (size_t)num_nodes * sizeof(double), s));
   NSM_HIP_C(hipMemsetAsync(md->status, 0, sizeof(int), s));
-  NSM_HIP_C(hipMemsetAsync(md->bc_code, 0, (size_t)num_nodes, s));
+  NSM_HIP_C(hipMemsetAsync(md->flags, 0, (size_t)num_nodes, s));
   if (E > 0) {
     // STEP_NP1 deformation gradient starts at the identity, stress at zero
-    // (src/nimble_kokkos_model_data.cc:921-924)
-    std::vector<char> ident((size_t)(72 * E) * ipw);
-    for (int64_t p = 0; p < 8 * E; p++)
-      for (int c = 0; c < 9; c++) {
-        const double val = c < 3 ? 1.0 : 0.0;
-        if (precision == NSM_FP64)
-          ((double*)ident.data())[p * 9 + c] = val;
-        else
-          ((float*)ident.data())[p * 9 + c] = (float)val;
-      }
+    // (src/nimble_kokkos_model_data.cc:921-924); [comp][ipt][E] layout, the
+    // diagonal components XX, YY, ZZ come first
+    std::vector<char> ident((size_t)(72 * E) * ipw, 0);
+    for (int64_t p = 0; p < 24 * E; p++) {
+      if (precision == NSM_FP64)
+        ((double*)ident.data())[p] = 1.0;
+      else
+        ((float*)ident.data())[p] = 1.0f;
+    }
     NSM_HIP_C(hipMemcpy(md->Fip, ident.data(), ident.size(), hipMemcpyHostToDevice));
     NSM_HIP_C(hipMemsetAsync(md->sip, 0, (size_t)(48 * E) * ipw, s));
   }
@@ -552,10 +574,11 @@ int nsm_model_destroy(nsm_model* md)
   free_dev(md->elem_tmp);
   free_dev(md->wg_min);
   free_dev(md->status);
-  free_dev(md->bc_code);
+  free_dev(md->flags);
   free_dev(md->bc_table);
-  free_dev(md->reaction_nodes);
   free_dev(md->reaction_hist);
+  free_dev(md->reaction_partials);
+  free_dev(md->reaction_tiles);
   for (auto& p : md->event_pool) {
     (void)hipEventDestroy(p.start);
     (void)hipEventDestroy(p.stop);
@@ -621,7 +644,7 @@ int nsm_model_compute_lumped_mass(nsm_model* md)
   if (!md->elem_tmp && (st = dalloc(&md->elem_tmp, std::max<size_t>((size_t)(8 * md->E), 1), &tmp))) return st;
   md->device_bytes += tmp;
   for (int b = 0; b < md->num_blocks; b++) {
-    nsm::launch_element_mass(md->blk_off[b], md->blk_off[b + 1], md->conn, md->X, md->rho[b], md->elem_tmp,
+    nsm::launch_element_mass(md->blk_off[b], md->blk_off[b + 1], md->E, md->conn, md->X, md->rho[b], md->elem_tmp,
                              md->status, md->stream);
     NSM_LAUNCHED();
   }
@@ -675,12 +698,8 @@ int nsm_model_compute_internal_force(nsm_model* md, double time_previous, double
   if (!md) return fail(NSM_ERR_ARG, "null model");
   int st = ensure_device(md);
   if (st || (st = launch_elements(md))) return st;
-  const nsm::NodeArrays na = md->node_arrays();
-  if (md->precision == NSM_FP64)
-    nsm::launch_force_gather<double>(na, (const double*)md->slots, md->stream);
-  else
-    nsm::launch_force_gather<float>(na, (const float*)md->slots, md->stream);
-  NSM_LAUNCHED();
+  nsm::StepTimes none{0.0, 0.0, 0.0, 0.0};
+  if ((st = launch_node(md, none, false, false, none, false, false, -1))) return st;
   return check_status(md);
 }
 
@@ -697,17 +716,13 @@ int nsm_model_set_reaction_nodes(nsm_model* md, int64_t num_nodes, const int* no
   if (num_nodes < 0 || (num_nodes > 0 && !node_ids)) return fail(NSM_ERR_ARG, "invalid node list");
   for (int64_t k = 0; k < num_nodes; k++)
     if (node_ids[k] < 0 || node_ids[k] >= md->N) return fail(NSM_ERR_ARG, "node id %d out of range", node_ids[k]);
-  int st = ensure_device(md);
-  if (st) return st;
-  free_dev(md->reaction_nodes);
-  md->num_reaction_nodes = 0;
-  if (num_nodes > 0) {
-    size_t tmp = 0;
-    if ((st = dalloc(&md->reaction_nodes, (size_t)num_nodes, &tmp))) return st;
-    NSM_HIP(hipMemcpy(md->reaction_nodes, node_ids, (size_t)num_nodes * sizeof(int), hipMemcpyHostToDevice));
-  }
+  // the set is kept as one flag bit per node (the node kernel sums flagged
+  // nodes); a node listed twice counts once
+  md->reaction_mask.assign((size_t)md->N, 0);
+  for (int64_t k = 0; k < num_nodes; k++) md->reaction_mask[(size_t)node_ids[k]] = 1;
   md->num_reaction_nodes = num_nodes;
   md->reaction_coord     = coordinate;
+  md->bc_dirty           = true;
   return NSM_OK;
 }
 
@@ -760,9 +775,9 @@ int nsm_model_explicit_steps(nsm_model* md, int64_t step_begin, int64_t step_end
     double         tn   = t;
     nsm::StepTimes nxt  = cur;
     if (!last) nxt = step_times(&tn, dt_user);
-    if ((st = launch_node(md, cur, is_output(step, num_load_steps, output_frequency), !last, nxt, last, true)))
+    if ((st = launch_node(md, cur, is_output(step, num_load_steps, output_frequency), !last, nxt, last, true,
+                          step)))
       return st;
-    if ((st = launch_reaction_at(md, step))) return st;
     if (!last) {
       t   = tn;
       cur = nxt;
@@ -789,19 +804,27 @@ int nsm_model_get_field(nsm_model* md, int field, double* out)
     case NSM_FIELD_LUMPED_MASS: src = md->m, cnt = (size_t)md->N; break;
     case NSM_FIELD_DEFORMATION_GRADIENT:
     case NSM_FIELD_STRESS: {
-      const size_t per = field == NSM_FIELD_DEFORMATION_GRADIENT ? 72 : 48;
+      // device [comp][ipt][E] -> host [E][ipt][comp] (the reference's element-data order)
+      const int    nc  = field == NSM_FIELD_DEFORMATION_GRADIENT ? 9 : 6;
       const void*  p   = field == NSM_FIELD_DEFORMATION_GRADIENT ? md->Fip : md->sip;
-      cnt              = per * (size_t)md->E;
+      const size_t E   = (size_t)md->E;
+      cnt              = (size_t)nc * 8 * E;
       if (cnt == 0) return NSM_OK;
+      std::vector<double> tmp(cnt);
       if (md->precision == NSM_FP64) {
-        NSM_HIP(hipMemcpyAsync(out, p, cnt * sizeof(double), hipMemcpyDeviceToHost, md->stream));
-      } else {
-        std::vector<float> tmp(cnt);
-        NSM_HIP(hipMemcpyAsync(tmp.data(), p, cnt * sizeof(float), hipMemcpyDeviceToHost, md->stream));
+        NSM_HIP(hipMemcpyAsync(tmp.data(), p, cnt * sizeof(double), hipMemcpyDeviceToHost, md->stream));
         NSM_HIP(hipStreamSynchronize(md->stream));
-        for (size_t i = 0; i < cnt; i++) out[i] = tmp[i];
+      } else {
+        std::vector<float> tf(cnt);
+        NSM_HIP(hipMemcpyAsync(tf.data(), p, cnt * sizeof(float), hipMemcpyDeviceToHost, md->stream));
+        NSM_HIP(hipStreamSynchronize(md->stream));
+        for (size_t i = 0; i < cnt; i++) tmp[i] = tf[i];
       }
-      NSM_HIP(hipStreamSynchronize(md->stream));
+      for (size_t c = 0; c < (size_t)nc; c++)
+        for (size_t q = 0; q < 8; q++) {
+          const double* src_row = tmp.data() + (c * 8 + q) * E;
+          for (size_t e = 0; e < E; e++) out[(e * 8 + q) * nc + c] = src_row[e];
+        }
       return NSM_OK;
     }
     default: return fail(NSM_ERR_ARG, "unknown field %d", field);

@@ -1,6 +1,6 @@
 // nsm_node.hip -- node-side kernels: deterministic CSR gather of the element
-// force slots fused with the central-difference update, kinematic
-// conditions, lumped-mass gather and the reaction-force sum.
+// force slots fused with the central-difference update and the reaction-force
+// sum, kinematic conditions, and the lumped-mass gather.
 //
 // Compiled with -ffp-contract=off: the velocity / displacement updates keep
 // the reference's separate multiply and add
@@ -17,10 +17,9 @@ constexpr int kNodeThreads = 256;
 
 // ApplyKinematicBC for the explicit scheme
 // (src/nimble_boundary_condition_manager.h:136-217), resolved per node.
-__device__ __forceinline__ void apply_bc(const NodeArrays& na, int64_t i, double v[3], const double u[3],
+__device__ __forceinline__ void apply_bc(const NodeArrays& na, int code, double v[3], const double u[3],
                                          double delta_t)
 {
-  const int code = na.bc_code[i];
   if (code == 0) return;
   const BcEntry& b = na.bc_table[code - 1];
 #pragma unroll
@@ -46,23 +45,7 @@ __device__ __forceinline__ void store3(double* p, int64_t i, const double r[3])
   p[3 * i + 2] = r[2];
 }
 
-template <typename SlotT>
-__device__ __forceinline__ void gather(const NodeArrays& na, const SlotT* __restrict__ slots, int64_t i, double f[3])
-{
-  const uint32_t p0 = na.node_ptr[i], p1 = na.node_ptr[i + 1];
-  double         f0 = 0.0, f1 = 0.0, f2 = 0.0;
-  for (uint32_t p = p0; p < p1; ++p) {
-    const SlotT* s = slots + 3 * (size_t)na.node_ent[p];
-    f0 += (double)s[0];
-    f1 += (double)s[1];
-    f2 += (double)s[2];
-  }
-  f[0] = f0;
-  f[1] = f1;
-  f[2] = f2;
-}
-
-// kick: v += (h * 1.0) * a ; drift: u += (dt * 1.0) * v
+// kick: v += (h * 1.0) * a  (also the drift u += (dt * 1.0) * v)
 __device__ __forceinline__ void kick(double v[3], const double a[3], double h)
 {
   const double prod = h * 1.0;
@@ -74,11 +57,13 @@ __device__ __forceinline__ void kick(double v[3], const double a[3], double h)
 __global__ __launch_bounds__(kNodeThreads) void k_apply_bc(NodeArrays na, StepTimes t)
 {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= na.num_nodes || na.bc_code[i] == 0) return;
+  if (i >= na.num_nodes) return;
+  const int code = na.flags[i] & 0x7f;
+  if (code == 0) return;
   double u[3], v[3];
   load3(na.u, i, u);
   load3(na.v, i, v);
-  apply_bc(na, i, v, u, t.t_cur - t.t_prev);
+  apply_bc(na, code, v, u, t.t_cur - t.t_prev);
   store3(na.v, i, v);
 }
 
@@ -87,63 +72,100 @@ __global__ __launch_bounds__(kNodeThreads) void k_prekick(NodeArrays na, StepTim
 {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= na.num_nodes) return;
-  double u[3], v[3], a[3];
+  const int code = na.flags[i] & 0x7f;
+  double    u[3], v[3], a[3];
   load3(na.u, i, u);
   load3(na.v, i, v);
   load3(na.a, i, a);
   const double delta_t = t.t_cur - t.t_prev;
   kick(v, a, t.half_dt);
-  apply_bc(na, i, v, u, delta_t);
+  apply_bc(na, code, v, u, delta_t);
   kick(u, v, t.dt);
-  apply_bc(na, i, v, u, delta_t);
+  apply_bc(na, code, v, u, delta_t);
   store3(na.u, i, u);
   store3(na.v, i, v);
 }
 
 // Internal-force gather and the second half of step n
 // (src/integrators/explicit_time_integrator.cc:250-270), optionally fused
-// with the first half of step n+1.
+// with the first half of step n+1, plus per-tile partial reaction sums over
+// flagged nodes.
 template <typename SlotT, bool kFuse>
 __global__ __launch_bounds__(kNodeThreads) void k_node_update(NodeArrays na, const SlotT* __restrict__ slots,
                                                               StepTimes cur, int output_bc, StepTimes nxt,
-                                                              int write_a, int update_v)
+                                                              int write_a, int update_v, ReactionArgs rx)
 {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= na.num_nodes) return;
-  double f[3];
-  gather(na, slots, i, f);
-  store3(na.f, i, f);
-  if (!update_v) return;
-
-  const double oneOverM = 1.0 / na.mass[i];
-  double       a[3], u[3], v[3];
-  a[0] = oneOverM * (f[0] + 0.0);
-  a[1] = oneOverM * (f[1] + 0.0);
-  a[2] = oneOverM * (f[2] + 0.0);
-  load3(na.u, i, u);
-  load3(na.v, i, v);
-  kick(v, a, cur.half_dt);
-  if (output_bc) apply_bc(na, i, v, u, cur.t_cur - cur.t_prev);
-  if (kFuse) {
-    const double delta_t = nxt.t_cur - nxt.t_prev;
-    kick(v, a, nxt.half_dt);
-    apply_bc(na, i, v, u, delta_t);
-    kick(u, v, nxt.dt);
-    apply_bc(na, i, v, u, delta_t);
-    store3(na.u, i, u);
+  const unsigned tile    = xcd_tile(blockIdx.x, gridDim.x);
+  const int64_t  i       = (int64_t)tile * kNodeThreads + threadIdx.x;
+  double         contrib = 0.0;
+  int            member  = 0;  // node belongs to the reaction set
+  if (i < na.num_nodes) {
+    // gather: ascending element order, starting from 0.0
+    const uint32_t p0 = na.node_ptr[i], p1 = na.node_ptr[i + 1];
+    double         f[3] = {0.0, 0.0, 0.0};
+    for (uint32_t p = p0; p < p1; ++p) {
+      const uint32_t ent = na.node_ent[p];
+      f[0] += (double)slots[ent];
+      f[1] += (double)slots[na.slot_stride + ent];
+      f[2] += (double)slots[2 * na.slot_stride + ent];
+    }
+    store3(na.f, i, f);
+    const int flags = na.flags[i];
+    if (rx.coord >= 0 && (flags & kReactionFlag)) {
+      member  = 1;
+      contrib = f[rx.coord];
+    }
+    if (update_v) {
+      const int    code     = flags & 0x7f;
+      const double oneOverM = 1.0 / na.mass[i];
+      double       a[3], u[3], v[3];
+      a[0] = oneOverM * (f[0] + 0.0);
+      a[1] = oneOverM * (f[1] + 0.0);
+      a[2] = oneOverM * (f[2] + 0.0);
+      load3(na.u, i, u);
+      load3(na.v, i, v);
+      kick(v, a, cur.half_dt);
+      if (output_bc) apply_bc(na, code, v, u, cur.t_cur - cur.t_prev);
+      if (kFuse) {
+        const double delta_t = nxt.t_cur - nxt.t_prev;
+        kick(v, a, nxt.half_dt);
+        apply_bc(na, code, v, u, delta_t);
+        kick(u, v, nxt.dt);
+        apply_bc(na, code, v, u, delta_t);
+        store3(na.u, i, u);
+      }
+      store3(na.v, i, v);
+      if (write_a) store3(na.a, i, a);
+    }
   }
-  store3(na.v, i, v);
-  if (write_a) store3(na.a, i, a);
+  if (rx.coord < 0) return;  // uniform across the grid
+
+  // partial of this node tile (fixed tree) when it holds reaction nodes
+  if (!__syncthreads_or(member)) return;
+  __shared__ double red[kNodeThreads];
+  red[threadIdx.x] = contrib;
+  __syncthreads();
+  for (int w = kNodeThreads / 2; w > 0; w >>= 1) {
+    if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) rx.partials[tile] = red[0];
 }
 
-template <typename SlotT>
-__global__ __launch_bounds__(kNodeThreads) void k_force_gather(NodeArrays na, const SlotT* __restrict__ slots)
+__global__ __launch_bounds__(kNodeThreads) void k_reaction_finalize(const double* __restrict__ partials,
+                                                                    const int* __restrict__ tiles, int num_tiles,
+                                                                    double* __restrict__ out)
 {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= na.num_nodes) return;
-  double f[3];
-  gather(na, slots, i, f);
-  store3(na.f, i, f);
+  __shared__ double red[kNodeThreads];
+  double            s = 0.0;
+  for (int k = threadIdx.x; k < num_tiles; k += kNodeThreads) s += partials[tiles[k]];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int w = kNodeThreads / 2; w > 0; w >>= 1) {
+    if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *out = -red[0];
 }
 
 __global__ __launch_bounds__(kNodeThreads) void k_mass_gather(int64_t N, const uint32_t* __restrict__ node_ptr,
@@ -158,76 +180,51 @@ __global__ __launch_bounds__(kNodeThreads) void k_mass_gather(int64_t N, const u
   m[i] = s;
 }
 
-// -sum f[node][coord] over a node list: fixed-order strided partial sums then
-// a fixed tree, so the result is reproducible run to run.
-__global__ __launch_bounds__(256) void k_reaction(const int* __restrict__ nodes, int64_t n, int coord,
-                                                  const double* __restrict__ f, double* __restrict__ out)
-{
-  __shared__ double part[256];
-  double            s = 0.0;
-  for (int64_t k = threadIdx.x; k < n; k += 256) s += f[3 * (int64_t)nodes[k] + coord];
-  part[threadIdx.x] = s;
-  __syncthreads();
-  for (int w = 128; w > 0; w >>= 1) {
-    if (threadIdx.x < w) part[threadIdx.x] += part[threadIdx.x + w];
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) *out = -part[0];
-}
-
-inline unsigned grid_of(int64_t n) { return (unsigned)((n + kNodeThreads - 1) / kNodeThreads); }
-
 }  // namespace
+
+unsigned node_grid(int64_t num_nodes) { return (unsigned)((num_nodes + kNodeThreads - 1) / kNodeThreads); }
 
 void launch_apply_bc(const NodeArrays& na, StepTimes t, hipStream_t s)
 {
   if (na.num_nodes <= 0) return;
-  hipLaunchKernelGGL(k_apply_bc, dim3(grid_of(na.num_nodes)), dim3(kNodeThreads), 0, s, na, t);
+  hipLaunchKernelGGL(k_apply_bc, dim3(node_grid(na.num_nodes)), dim3(kNodeThreads), 0, s, na, t);
 }
 
 void launch_prekick(const NodeArrays& na, StepTimes t, hipStream_t s)
 {
   if (na.num_nodes <= 0) return;
-  hipLaunchKernelGGL(k_prekick, dim3(grid_of(na.num_nodes)), dim3(kNodeThreads), 0, s, na, t);
+  hipLaunchKernelGGL(k_prekick, dim3(node_grid(na.num_nodes)), dim3(kNodeThreads), 0, s, na, t);
 }
 
 template <typename SlotT>
 void launch_node_update(const NodeArrays& na, const SlotT* slots, StepTimes cur, bool output_bc, bool fuse_next,
-                        StepTimes next, bool write_a, bool update_v, hipStream_t s)
+                        StepTimes next, bool write_a, bool update_v, ReactionArgs rx, hipStream_t s)
 {
   if (na.num_nodes <= 0) return;
   if (fuse_next)
-    hipLaunchKernelGGL((k_node_update<SlotT, true>), dim3(grid_of(na.num_nodes)), dim3(kNodeThreads), 0, s, na,
-                       slots, cur, (int)output_bc, next, (int)write_a, (int)update_v);
+    hipLaunchKernelGGL((k_node_update<SlotT, true>), dim3(node_grid(na.num_nodes)), dim3(kNodeThreads), 0, s, na,
+                       slots, cur, (int)output_bc, next, (int)write_a, (int)update_v, rx);
   else
-    hipLaunchKernelGGL((k_node_update<SlotT, false>), dim3(grid_of(na.num_nodes)), dim3(kNodeThreads), 0, s, na,
-                       slots, cur, (int)output_bc, next, (int)write_a, (int)update_v);
-}
-
-template <typename SlotT>
-void launch_force_gather(const NodeArrays& na, const SlotT* slots, hipStream_t s)
-{
-  if (na.num_nodes <= 0) return;
-  hipLaunchKernelGGL((k_force_gather<SlotT>), dim3(grid_of(na.num_nodes)), dim3(kNodeThreads), 0, s, na, slots);
+    hipLaunchKernelGGL((k_node_update<SlotT, false>), dim3(node_grid(na.num_nodes)), dim3(kNodeThreads), 0, s, na,
+                       slots, cur, (int)output_bc, next, (int)write_a, (int)update_v, rx);
 }
 
 template void launch_node_update<double>(const NodeArrays&, const double*, StepTimes, bool, bool, StepTimes, bool,
-                                         bool, hipStream_t);
+                                         bool, ReactionArgs, hipStream_t);
 template void launch_node_update<float>(const NodeArrays&, const float*, StepTimes, bool, bool, StepTimes, bool, bool,
-                                        hipStream_t);
-template void launch_force_gather<double>(const NodeArrays&, const double*, hipStream_t);
-template void launch_force_gather<float>(const NodeArrays&, const float*, hipStream_t);
+                                        ReactionArgs, hipStream_t);
+
+void launch_reaction_finalize(const double* partials, const int* tiles, int num_tiles, double* out, hipStream_t s)
+{
+  hipLaunchKernelGGL(k_reaction_finalize, dim3(1), dim3(kNodeThreads), 0, s, partials, tiles, num_tiles, out);
+}
 
 void launch_mass_gather(int64_t N, const uint32_t* node_ptr, const uint32_t* node_ent, const double* elem_mass,
                         double* m, hipStream_t s)
 {
   if (N <= 0) return;
-  hipLaunchKernelGGL(k_mass_gather, dim3(grid_of(N)), dim3(kNodeThreads), 0, s, N, node_ptr, node_ent, elem_mass, m);
-}
-
-void launch_reaction(const int* nodes, int64_t n, int coord, const double* f, double* out, hipStream_t s)
-{
-  hipLaunchKernelGGL(k_reaction, dim3(1), dim3(256), 0, s, nodes, n, coord, f, out);
+  hipLaunchKernelGGL(k_mass_gather, dim3(node_grid(N)), dim3(kNodeThreads), 0, s, N, node_ptr, node_ent, elem_mass,
+                     m);
 }
 
 }  // namespace nsm

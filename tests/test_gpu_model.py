@@ -192,3 +192,24 @@ def test_large_cube_properties(hip_device):
     assert rel_err(res["mixed"][0], res["fp64"][0]) <= 1e-6
     assert rel_err(res["mixed"][1], res["fp64"][1]) <= 1e-5
     assert rel_err(res["mixed"][2], res["fp64"][2]) <= 1e-5
+
+
+def test_reaction_over_scattered_nodes(hip_device):
+    """Reaction sum over an arbitrary node set (many node tiles, some nodes
+    with zero force in early steps) against the oracle's serial sum."""
+    prob = N.CubeProblem(n=12, materials=[N.problems.STEEL, N.problems.ALUMINIUM], num_steps=40)
+    rng = np.random.default_rng(3)
+    nodes = np.sort(rng.choice(prob.num_nodes, 300, replace=False)).astype(np.int32)
+    for precision, tol in (("fp64", 1e-13), ("mixed", 1e-4)):
+        m = prob.make_model(precision)
+        m.SetReactionNodes(nodes, 0)
+        integ = prob.make_integrator(m)
+        integ.Integrate()
+        om = oracle_model_from_problem(prob)
+        dt = prob.dt_factor * om.critical_dt()
+        run = O.ExplicitRun(om, prob.num_steps * dt, prob.num_steps, 0, reaction_nodes=nodes, reaction_coord=0)
+        run.advance(prob.num_steps)
+        R = m.GetReactionHistory(0, prob.num_steps)
+        scale = np.abs(run.reaction).max()
+        assert np.abs(R - run.reaction).max() <= tol * scale, (precision, np.abs(R - run.reaction).max() / scale)
+        m.close()
