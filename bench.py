@@ -1,9 +1,9 @@
 """Benchmark: explicit internal-force step on the 128^3 two-block HEX8 cube.
 
 One step = one central-difference step of the whole mesh on the device:
-fused element kernel (F -> Neo-Hookean stress -> B^T.sigma) per block, CSR
-node gather fused with the velocity / displacement update, and the top-face
-reaction sum.  Inputs are resident in HBM before timing starts.
+fused element kernel (F -> Neo-Hookean stress -> B^T.sigma) over all blocks,
+CSR node gather fused with the velocity / displacement update, and the
+top-face reaction sum.  Inputs are resident in HBM before timing starts.
 
     python bench.py [--gpus 1] [--steps 500] [--warmup 20] [--precision mixed|fp64]
                     [--workload cube128_2blk] [--cpu-steps 8] [--no-cpu]
@@ -31,14 +31,21 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
 
 
+MAX_BLOCKS_PER_LAUNCH = 8  # element blocks sharing one element-kernel launch
+
+
 def algorithmic_bytes(prob, s_bytes):
-    """Entry-point byte model.  Per block launch of the element kernel:
-    E_b*(32 + 120*s) + N_b*48 (N_b = distinct nodes of the block); per step:
-    E*(32 + 120*s) + N*152."""
+    """Entry-point byte model (s = bytes per integration-point word).
+    Element kernel, per launch over a group of blocks: E_g*(32 + 120*s) + N_g*48
+    (int32 connectivity; F and sigma written once per Gauss point; X and u read
+    once per distinct node N_g of the group).  Whole step:
+    E*(32 + 120*s) + N*152 (adds v, m read and u, v, f_int written per node)."""
+    ranges = prob.block_ranges()
     per_launch = []
-    for (e0, e1) in prob.block_ranges():
-        nb = len(np.unique(prob.conn[e0:e1]))
-        per_launch.append((e1 - e0) * (32 + 120 * s_bytes) + nb * 48)
+    for g in range(0, len(ranges), MAX_BLOCKS_PER_LAUNCH):
+        e0, e1 = ranges[g][0], ranges[min(g + MAX_BLOCKS_PER_LAUNCH, len(ranges)) - 1][1]
+        ng = len(np.unique(prob.conn[e0:e1]))
+        per_launch.append((e1 - e0) * (32 + 120 * s_bytes) + ng * 48)
     step = prob.num_elem * (32 + 120 * s_bytes) + prob.num_nodes * 152
     return per_launch, step
 
@@ -86,6 +93,8 @@ def main():
     ap.add_argument("--workload", default="cube128_2blk")
     ap.add_argument("--cpu-steps", type=int, default=8)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run 5 device copies of MIB MiB (calibrates rocprofv3 FETCH_SIZE / WRITE_SIZE)")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -101,6 +110,16 @@ def main():
         print("bench.py: no HIP device visible", file=sys.stderr)
         sys.exit(1)
     torch.cuda.init()
+
+    if args.copy_calibration:
+        nbytes = args.copy_calibration << 20
+        src = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calibration_bytes": nbytes, "copies": 5}), flush=True)
+        return
 
     prob = N.workload(args.workload).build()
     s_bytes = 4 if args.precision == "mixed" else 8
@@ -132,7 +151,7 @@ def main():
 
     per_launch, step_bytes = algorithmic_bytes(prob, s_bytes)
     launches_per_step = len(per_launch)
-    # mean algorithmic bytes per launch over the launches timed (blocks alternate)
+    # mean algorithmic bytes per launch over the launches of one step
     bytes_per_launch = float(np.mean(per_launch))
     achieved = bytes_per_launch / (el_ms * 1e-3) / 1e9 if el_ms > 0 else 0.0
     traffic = load_traffic(args.workload, args.precision)

@@ -171,18 +171,20 @@ struct Lds {
   };
 };
 
-__global__ __launch_bounds__(kThreads) void k_element_fp64(int64_t e_begin, int64_t e_end, int64_t E_total,
+__global__ __launch_bounds__(kThreads) void k_element_fp64(const BlockTable<double> bt, int64_t E_total,
                                                            const int* __restrict__ conn,
                                                            const double* __restrict__ X,
-                                                           const double* __restrict__ u, double K, double G,
+                                                           const double* __restrict__ u,
                                                            double* __restrict__ Fip, double* __restrict__ sip,
                                                            double* __restrict__ slots, int* __restrict__ status)
 {
   __shared__ Lds lds;
   const int     el    = threadIdx.x >> 3;
   const int     k     = threadIdx.x & 7;
-  const int64_t e     = e_begin + (int64_t)xcd_tile(blockIdx.x, gridDim.x) * kEPW + el;
-  const bool    valid = e < e_end;
+  const int64_t e     = bt.off[0] + (int64_t)xcd_tile(blockIdx.x, gridDim.x) * kEPW + el;
+  const bool    valid = e < bt.off[bt.count];
+  const int     blk   = valid ? block_of(bt, e) : 0;
+  const double  K     = bt.K[blk], G = bt.G[blk];
 
   if (valid) {
     const int64_t nd = conn[e * 8 + k];
@@ -442,13 +444,13 @@ inline unsigned grid_of(int64_t n, int64_t per) { return (unsigned)((n + per - 1
 
 }  // namespace
 
-void launch_element_fp64(int64_t e_begin, int64_t e_end, int64_t E_total, const int* conn, const double* X,
-                         const double* u, double K, double G, double* Fip, double* sip, double* slots, int* status,
-                         hipStream_t s)
+void launch_element_fp64(const BlockTable<double>& bt, int64_t E_total, const int* conn, const double* X,
+                         const double* u, double* Fip, double* sip, double* slots, int* status, hipStream_t s)
 {
-  if (e_end <= e_begin) return;
-  hipLaunchKernelGGL(k_element_fp64, dim3(grid_of(e_end - e_begin, kEPW)), dim3(kThreads), 0, s, e_begin, e_end,
-                     E_total, conn, X, u, K, G, Fip, sip, slots, status);
+  const int64_t n = bt.off[bt.count] - bt.off[0];
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_element_fp64, dim3(grid_of(n, kEPW)), dim3(kThreads), 0, s, bt, E_total, conn, X, u, Fip, sip,
+                     slots, status);
 }
 
 void launch_seam_defgrad(int64_t E, const double* Xg, const double* ug, double* F, int* status, hipStream_t s)

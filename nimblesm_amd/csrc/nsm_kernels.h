@@ -63,13 +63,31 @@ __device__ __forceinline__ unsigned xcd_tile(unsigned b, unsigned nb)
   return x * q + (x < r ? x : r) + k;
 }
 
-// ---- element kernels (element range [e_begin, e_end) of one block) ----
-void launch_element_fp64(int64_t e_begin, int64_t e_end, int64_t E_total, const int* conn, const double* X,
-                         const double* u, double K, double G, double* Fip, double* sip, double* slots, int* status,
-                         hipStream_t s);
-void launch_element_mixed(int64_t e_begin, int64_t e_end, int64_t E_total, const int* conn, const double* X,
-                          const double* u, float K, float G, float* Fip, float* sip, float* slots, int* status,
-                          hipStream_t s);
+// Up to kMaxLaunchBlocks consecutive element blocks handled by one launch:
+// block b owns elements [off[b], off[b+1]) with moduli K[b], G[b].
+constexpr int kMaxLaunchBlocks = 8;
+template <typename Real>
+struct BlockTable {
+  int     count;
+  int64_t off[kMaxLaunchBlocks + 1];
+  Real    K[kMaxLaunchBlocks];
+  Real    G[kMaxLaunchBlocks];
+};
+
+// block of element e (blocks ascending, e inside the table's range)
+template <typename Real>
+__device__ __forceinline__ int block_of(const BlockTable<Real>& bt, int64_t e)
+{
+  int b = 0;
+  while (b + 1 < bt.count && e >= bt.off[b + 1]) b++;
+  return b;
+}
+
+// ---- element kernels over the elements of a block table ----
+void launch_element_fp64(const BlockTable<double>& bt, int64_t E_total, const int* conn, const double* X,
+                         const double* u, double* Fip, double* sip, double* slots, int* status, hipStream_t s);
+void launch_element_mixed(const BlockTable<float>& bt, int64_t E_total, const int* conn, const double* X,
+                          const double* u, float* Fip, float* sip, float* slots, int* status, hipStream_t s);
 
 // ---- element / material seams on gathered arrays ----
 void launch_seam_defgrad(int64_t E, const double* Xg, const double* ug, double* F, int* status, hipStream_t s);

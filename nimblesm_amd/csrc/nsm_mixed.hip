@@ -25,15 +25,15 @@ constexpr int kThreads = 256;
 #define NSM_MIXED_WAVES_ATTR
 #endif
 
-__global__ __launch_bounds__(kThreads) NSM_MIXED_WAVES_ATTR void k_element_mixed(int64_t e_begin, int64_t e_end, int64_t E_total,
-                                                            const int* __restrict__ conn,
-                                                            const double* __restrict__ X,
-                                                            const double* __restrict__ u, float K, float G,
-                                                            float* __restrict__ Fip, float* __restrict__ sip,
-                                                            float* __restrict__ slots, int* __restrict__ status)
+__global__ __launch_bounds__(kThreads) NSM_MIXED_WAVES_ATTR void k_element_mixed(
+    const BlockTable<float> bt, int64_t E_total, const int* __restrict__ conn, const double* __restrict__ X,
+    const double* __restrict__ u, float* __restrict__ Fip, float* __restrict__ sip, float* __restrict__ slots,
+    int* __restrict__ status)
 {
-  const int64_t e = e_begin + (int64_t)xcd_tile(blockIdx.x, gridDim.x) * kThreads + threadIdx.x;
-  if (e >= e_end) return;
+  const int64_t e = bt.off[0] + (int64_t)xcd_tile(blockIdx.x, gridDim.x) * kThreads + threadIdx.x;
+  if (e >= bt.off[bt.count]) return;
+  const int   b = block_of(bt, e);
+  const float K = bt.K[b], G = bt.G[b];
 
   const int4 c0 = *reinterpret_cast<const int4*>(conn + 8 * e);
   const int4 c1 = *reinterpret_cast<const int4*>(conn + 8 * e + 4);
@@ -66,14 +66,14 @@ __global__ __launch_bounds__(kThreads) NSM_MIXED_WAVES_ATTR void k_element_mixed
 
 }  // namespace
 
-void launch_element_mixed(int64_t e_begin, int64_t e_end, int64_t E_total, const int* conn, const double* X,
-                          const double* u, float K, float G, float* Fip, float* sip, float* slots, int* status,
-                          hipStream_t s)
+void launch_element_mixed(const BlockTable<float>& bt, int64_t E_total, const int* conn, const double* X,
+                          const double* u, float* Fip, float* sip, float* slots, int* status, hipStream_t s)
 {
-  if (e_end <= e_begin) return;
-  const unsigned grid = (unsigned)((e_end - e_begin + kThreads - 1) / kThreads);
-  hipLaunchKernelGGL(k_element_mixed, dim3(grid), dim3(kThreads), 0, s, e_begin, e_end, E_total, conn, X, u, K, G,
-                     Fip, sip, slots, status);
+  const int64_t n = bt.off[bt.count] - bt.off[0];
+  if (n <= 0) return;
+  const unsigned grid = (unsigned)((n + kThreads - 1) / kThreads);
+  hipLaunchKernelGGL(k_element_mixed, dim3(grid), dim3(kThreads), 0, s, bt, E_total, conn, X, u, Fip, sip, slots,
+                     status);
 }
 
 }  // namespace nsm

@@ -287,23 +287,38 @@ int check_status(nsm_model* md)
   return NSM_OK;
 }
 
+// All blocks in one launch when they fit one block table, else groups of
+// kMaxLaunchBlocks consecutive blocks.
 int launch_elements(nsm_model* md)
 {
-  for (int b = 0; b < md->num_blocks; b++) {
-    size_t ti;
-    int    st = time_begin(md, NSM_KERNEL_ELEMENT, &ti);
+  for (int b0 = 0; b0 < md->num_blocks; b0 += nsm::kMaxLaunchBlocks) {
+    const int nb = std::min(nsm::kMaxLaunchBlocks, md->num_blocks - b0);
+    size_t    ti;
+    int       st = time_begin(md, NSM_KERNEL_ELEMENT, &ti);
     if (st) return st;
-    if (md->precision == NSM_FP64)
-      nsm::launch_element_fp64(md->blk_off[b], md->blk_off[b + 1], md->E, md->conn, md->X, md->u, md->K[b],
-                               md->G[b], (double*)md->Fip, (double*)md->sip, (double*)md->slots, md->status,
-                               md->stream);
-    else
-      nsm::launch_element_mixed(md->blk_off[b], md->blk_off[b + 1], md->E, md->conn, md->X, md->u,
-                                (float)md->K[b], (float)md->G[b], (float*)md->Fip, (float*)md->sip,
+    if (md->precision == NSM_FP64) {
+      nsm::BlockTable<double> bt{};
+      bt.count = nb;
+      for (int b = 0; b <= nb; b++) bt.off[b] = md->blk_off[b0 + b];
+      for (int b = 0; b < nb; b++) {
+        bt.K[b] = md->K[b0 + b];
+        bt.G[b] = md->G[b0 + b];
+      }
+      nsm::launch_element_fp64(bt, md->E, md->conn, md->X, md->u, (double*)md->Fip, (double*)md->sip,
+                               (double*)md->slots, md->status, md->stream);
+    } else {
+      nsm::BlockTable<float> bt{};
+      bt.count = nb;
+      for (int b = 0; b <= nb; b++) bt.off[b] = md->blk_off[b0 + b];
+      for (int b = 0; b < nb; b++) {
+        bt.K[b] = (float)md->K[b0 + b];
+        bt.G[b] = (float)md->G[b0 + b];
+      }
+      nsm::launch_element_mixed(bt, md->E, md->conn, md->X, md->u, (float*)md->Fip, (float*)md->sip,
                                 (float*)md->slots, md->status, md->stream);
+    }
     NSM_LAUNCHED();
-    st = time_end(md, ti);
-    if (st) return st;
+    if ((st = time_end(md, ti))) return st;
   }
   return NSM_OK;
 }

@@ -100,14 +100,31 @@ __global__ __launch_bounds__(kNodeThreads) void k_node_update(NodeArrays na, con
   double         contrib = 0.0;
   int            member  = 0;  // node belongs to the reaction set
   if (i < na.num_nodes) {
-    // gather: ascending element order, starting from 0.0
+    // gather: ascending element order, starting from 0.0; incidences are
+    // fetched 8 at a time so their loads overlap, then added in order
     const uint32_t p0 = na.node_ptr[i], p1 = na.node_ptr[i + 1];
     double         f[3] = {0.0, 0.0, 0.0};
-    for (uint32_t p = p0; p < p1; ++p) {
-      const uint32_t ent = na.node_ent[p];
-      f[0] += (double)slots[ent];
-      f[1] += (double)slots[na.slot_stride + ent];
-      f[2] += (double)slots[2 * na.slot_stride + ent];
+    for (uint32_t p = p0; p < p1; p += 8) {
+      const uint32_t n = p1 - p < 8u ? p1 - p : 8u;
+      uint32_t       ent[8];
+      SlotT          s0[8], s1[8], s2[8];
+#pragma unroll
+      for (uint32_t k = 0; k < 8; k++)
+        if (k < n) ent[k] = na.node_ent[p + k];
+#pragma unroll
+      for (uint32_t k = 0; k < 8; k++)
+        if (k < n) {
+          s0[k] = slots[ent[k]];
+          s1[k] = slots[na.slot_stride + ent[k]];
+          s2[k] = slots[2 * na.slot_stride + ent[k]];
+        }
+#pragma unroll
+      for (uint32_t k = 0; k < 8; k++)
+        if (k < n) {
+          f[0] += (double)s0[k];
+          f[1] += (double)s1[k];
+          f[2] += (double)s2[k];
+        }
     }
     store3(na.f, i, f);
     const int flags = na.flags[i];
