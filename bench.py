@@ -1,18 +1,19 @@
 """Benchmark: explicit internal-force step on the 128^3 two-block HEX8 cube.
 
 One step = one central-difference step of the whole mesh on the device:
-fused element kernel (F -> Neo-Hookean stress -> B^T.sigma) over all blocks,
-CSR node gather fused with the velocity / displacement update, and the
-top-face reaction sum.  Inputs are resident in HBM before timing starts.
+fused element kernel (F -> Neo-Hookean stress -> B^T.sigma) over all blocks
+with the nodal force sum and update (mixed precision: tile-local assembly in
+LDS, then the nodes shared between tiles; FP64: element force slots, then a
+CSR node gather), and the top-face reaction sum.  Inputs are resident in HBM
+before timing starts.
 
     python bench.py [--gpus 1] [--steps 500] [--warmup 20] [--precision mixed|fp64]
                     [--workload cube128_2blk] [--cpu-steps 8] [--no-cpu]
 
 Prints one JSON line.  `value` is element-steps/s of the timed K steps.
-`roofline` covers the dominant (element) kernel: algorithmic bytes per launch
-(connectivity + F and sigma written once per Gauss point + X and u read once
-per node) over its average launch duration from HIP events on the model's
-stream.  `cpu_baseline` times the FP64 CPU oracle (OpenMP, deterministic
+`roofline` covers the dominant (element) kernel: its algorithmic bytes per
+launch (algorithmic_bytes below) over its average launch duration from HIP
+events on the model's stream.  `cpu_baseline` times the FP64 CPU oracle (OpenMP, deterministic
 gather) on a bounded sample of the same workload.
 """
 from __future__ import annotations
@@ -34,18 +35,30 @@ HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
 MAX_BLOCKS_PER_LAUNCH = 8  # element blocks sharing one element-kernel launch
 
 
-def algorithmic_bytes(prob, s_bytes):
-    """Entry-point byte model (s = bytes per integration-point word).
-    Element kernel, per launch over a group of blocks: E_g*(32 + 120*s) + N_g*48
-    (int32 connectivity; F and sigma written once per Gauss point; X and u read
-    once per distinct node N_g of the group).  Whole step:
-    E*(32 + 120*s) + N*152 (adds v, m read and u, v, f_int written per node)."""
+def algorithmic_bytes(prob, s_bytes, assembly):
+    """Entry-point byte model (s = bytes per integration-point word), per step:
+    E*(32 + 120*s) + N*152 -- int32 connectivity, F and sigma written once per
+    Gauss point, and per node X, u, v, m read and u, v, f_int written once.
+
+    Split over the launches that do the work:
+      slot path, element kernel per group of blocks: E_g*(32 + 120*s) + N_g*48
+        (X and u of the N_g distinct nodes of the group); the node kernel
+        carries the remaining N*104;
+      tiled path, element kernel: E*(32 + 120*s) + N_int*152 + N_bnd*48 (it
+        completes the step of the N_int tile-interior nodes and reads X, u of
+        the others); the boundary kernel carries N_bnd*104."""
     ranges = prob.block_ranges()
     per_launch = []
-    for g in range(0, len(ranges), MAX_BLOCKS_PER_LAUNCH):
-        e0, e1 = ranges[g][0], ranges[min(g + MAX_BLOCKS_PER_LAUNCH, len(ranges)) - 1][1]
-        ng = len(np.unique(prob.conn[e0:e1]))
-        per_launch.append((e1 - e0) * (32 + 120 * s_bytes) + ng * 48)
+    if assembly["tiled"]:
+        if len(ranges) > MAX_BLOCKS_PER_LAUNCH:
+            raise SystemExit("bench.py: the tiled byte split assumes one element launch (at most 8 blocks)")
+        n_int, n_bnd = assembly["interior_nodes"], assembly["boundary_nodes"]
+        per_launch.append(prob.num_elem * (32 + 120 * s_bytes) + n_int * 152 + n_bnd * 48)
+    else:
+        for g in range(0, len(ranges), MAX_BLOCKS_PER_LAUNCH):
+            e0, e1 = ranges[g][0], ranges[min(g + MAX_BLOCKS_PER_LAUNCH, len(ranges)) - 1][1]
+            ng = len(np.unique(prob.conn[e0:e1]))
+            per_launch.append((e1 - e0) * (32 + 120 * s_bytes) + ng * 48)
     step = prob.num_elem * (32 + 120 * s_bytes) + prob.num_nodes * 152
     return per_launch, step
 
@@ -149,7 +162,8 @@ def main():
     rx_ms, rx_n = model.KernelTime("reaction")
     model.EnableKernelTiming(False)
 
-    per_launch, step_bytes = algorithmic_bytes(prob, s_bytes)
+    assembly = model.AssemblyInfo()
+    per_launch, step_bytes = algorithmic_bytes(prob, s_bytes, assembly)
     launches_per_step = len(per_launch)
     # mean algorithmic bytes per launch over the launches of one step
     bytes_per_launch = float(np.mean(per_launch))
@@ -178,6 +192,7 @@ def main():
             "nodes": prob.num_nodes,
             "blocks": len(prob.materials),
             "precision": args.precision,
+            "assembly": "tiles" if assembly["tiled"] else "slots",
             "steps": args.steps,
         },
         "roofline": {
@@ -192,6 +207,8 @@ def main():
             "element_ms_avg": el_ms, "element_launches": el_n, "element_launches_per_step": launches_per_step,
             "element_alg_bytes_per_launch": bytes_per_launch,
             "node_ms_avg": nd_ms, "node_launches": nd_n,
+            "tiles": assembly["num_tiles"], "interior_nodes": assembly["interior_nodes"],
+            "boundary_nodes": assembly["boundary_nodes"],
             "reaction_ms_avg": rx_ms, "reaction_launches": rx_n,
         },
         "step_roofline": {

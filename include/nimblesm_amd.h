@@ -60,8 +60,10 @@ typedef enum nsm_bc_kind {
 
 /* kernels whose launch durations nsm_model_kernel_time() reports */
 typedef enum nsm_kernel {
-  NSM_KERNEL_ELEMENT = 0, /* fused F -> stress -> B^T.sigma element kernel      */
-  NSM_KERNEL_NODE    = 1, /* CSR gather + central-difference node kernel         */
+  NSM_KERNEL_ELEMENT = 0, /* fused F -> stress -> B^T.sigma element kernel
+                             (tiled: also the update of tile-interior nodes)   */
+  NSM_KERNEL_NODE    = 1, /* CSR gather + central-difference node kernel
+                             (tiled: update of nodes shared between tiles)     */
   NSM_KERNEL_REACTION = 2 /* reaction-force sum                                  */
 } nsm_kernel;
 
@@ -168,8 +170,24 @@ int nsm_model_explicit_steps(nsm_model* model, int64_t step_begin, int64_t step_
 int nsm_model_get_field(nsm_model* model, int field, double* out);
 int nsm_model_set_field(nsm_model* model, int field, const double* in);
 /* Device address of a field (node fields double; ipt fields double in FP64
- * mode and float in mixed mode) and its element size in bytes. */
+ * mode and float in mixed mode) and its element size in bytes.  Ipt fields
+ * are element-fastest [comp][ipt][E] in the element order of
+ * nsm_model_get_element_order. */
 int nsm_model_field_device_pointer(nsm_model* model, int field, void** ptr, int* elem_bytes);
+/* Element order of the device ipt arrays: out[i] (i < E) is the element, as
+ * numbered by the conn given to nsm_model_create, stored at device position
+ * i.  FP64 models keep that order; tiled mixed models store each block's
+ * elements tile by tile.  No reference counterpart: the reference's views
+ * keep the mesh order (src/nimble_kokkos_model_data.cc:894-961). */
+int nsm_model_get_element_order(nsm_model* model, int64_t* out);
+/* How the model assembles nodal forces: *tiled = 1 for tile-local assembly
+ * (mixed precision), with the number of element tiles, of nodes updated
+ * inside their tile and of nodes updated from per-tile partial sums;
+ * *tiled = 0 for element force slots + CSR node gather (all nodes "boundary").
+ * No reference counterpart (the reference scatters element forces,
+ * src/nimble_kokkos_model_data.cc:1259-1280). */
+int nsm_model_assembly_info(nsm_model* model, int* tiled, int64_t* num_tiles, int64_t* interior_nodes,
+                            int64_t* boundary_nodes);
 
 int nsm_model_synchronize(nsm_model* model);
 /* Launch-duration statistics gathered by HIP events around every launch of

@@ -28,6 +28,8 @@ SOURCES = {
     "nsm_fp64.hip": ["-ffp-contract=off"],
     "nsm_node.hip": ["-ffp-contract=off"],
     "nsm_mixed.hip": ["-ffp-contract=fast"],
+    "nsm_tiled.hip": ["-ffp-contract=fast"],
+    "nsm_tiles.cpp": ["-ffp-contract=off", "-x", "hip"],
     "nsm_model.cpp": ["-ffp-contract=off", "-x", "hip"],
 }
 

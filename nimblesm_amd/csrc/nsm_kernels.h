@@ -45,6 +45,7 @@ struct NodeArrays {
   double*         f;            // [N][3] internal force
   const uint8_t*  flags;        // [N]
   const BcEntry*  bc_table;
+  int             num_bc;       // entries of bc_table
 };
 
 // Reaction-force sum: the node kernel writes one partial per node tile
@@ -114,12 +115,12 @@ unsigned node_grid(int64_t num_nodes);
 void launch_apply_bc(const NodeArrays& na, StepTimes t, hipStream_t s);
 // first half kick + BC + drift + BC of one step
 void launch_prekick(const NodeArrays& na, StepTimes t, hipStream_t s);
+struct StepArgs;
 // gather f from element slots, a = f/m, second half kick (+BC on output
-// steps); if fuse_next, also the next step's first half kick, BC, drift, BC;
-// optional reaction sum.
+// steps); if fused, also the next step's first half kick, BC, drift, BC;
+// optional per-tile reaction partials.
 template <typename SlotT>
-void launch_node_update(const NodeArrays& na, const SlotT* slots, StepTimes cur, bool output_bc, bool fuse_next,
-                        StepTimes next, bool write_a, bool update_v, ReactionArgs rx, hipStream_t s);
+void launch_node_update(const NodeArrays& na, const SlotT* slots, const StepArgs& sa, ReactionArgs rx, hipStream_t s);
 // out = -sum_k partials[tiles[k]], k ascending (fixed tree, reproducible)
 void launch_reaction_finalize(const double* partials, const int* tiles, int num_tiles, double* out, hipStream_t s);
 

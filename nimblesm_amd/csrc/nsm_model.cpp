@@ -2,14 +2,18 @@
 // the explicit central-difference driver and the element / material seams.
 //
 // The model keeps every node and integration-point array in HBM for its whole
-// life; a step is a short sequence of stream-ordered launches (element kernel
-// per block, node kernel, optional reaction sum) with no host round trip.
+// life; a step is a short sequence of stream-ordered launches with no host
+// round trip: FP64 runs the element kernel (force slots) then the CSR node
+// kernel; mixed precision runs the tiled element kernel (LDS assembly and the
+// update of tile-interior nodes) then the boundary-node kernel; either ends
+// with the optional reaction sum.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <mutex>
@@ -18,6 +22,9 @@
 
 #include "../../include/nimblesm_amd.h"
 #include "nsm_kernels.h"
+#include "nsm_node_step.h"
+#include "nsm_tiled.h"
+#include "nsm_tiles.h"
 
 namespace {
 
@@ -133,10 +140,18 @@ struct nsm_model {
   int       reaction_coord = 2;
   double*   reaction_hist = nullptr;
   int64_t   reaction_cap = 0;
-  double*   reaction_partials = nullptr;  // [node tiles]
-  int*      reaction_tiles = nullptr;     // node tiles holding reaction nodes, ascending
+  double*   reaction_partials = nullptr;  // one per workgroup of the kernels that sum reactions
+  int*      reaction_tiles = nullptr;     // partials holding reaction nodes, ascending
   int       num_reaction_tiles = 0;
   size_t    device_bytes = 0;
+
+  // tile-local assembly (mixed precision): element tiles, device copies of
+  // their arrays; integration-point arrays are stored in tile element order
+  bool                 tiled = false;
+  nsm::TileSet         tiles;
+  std::vector<int>     blk_tile_off;  // first tile of each block, [num_blocks + 1]
+  nsm::TileArgs        tile_args{};
+  std::vector<void*>   tile_bufs;
 
   std::vector<BcRecord> bcs;
   std::vector<uint8_t>  reaction_mask;  // host copy of the reaction bit per node
@@ -164,6 +179,7 @@ struct nsm_model {
     na.f         = f;
     na.flags     = flags;
     na.bc_table  = bc_table;
+    na.num_bc    = num_bc_entries;
     return na;
   }
 };
@@ -253,12 +269,29 @@ int rebuild_bcs(nsm_model* md)
   if (!table.empty())
     NSM_HIP(hipMemcpy(md->bc_table, table.data(), table.size() * sizeof(nsm::BcEntry), hipMemcpyHostToDevice));
   std::vector<int> tiles;
+  const bool       have_rx = md->reaction_mask.size() == code.size();
   for (size_t i = 0; i < code.size(); i++)
-    if (md->reaction_mask.size() == code.size() && md->reaction_mask[i]) {
+    if (have_rx && md->reaction_mask[i]) {
       code[i] |= nsm::kReactionFlag;
       const int t = (int)(i / 256);  // node tile of the node kernel
-      if (tiles.empty() || tiles.back() != t) tiles.push_back(t);
+      if (!md->tiled && (tiles.empty() || tiles.back() != t)) tiles.push_back(t);
     }
+  if (md->tiled && have_rx) {
+    // element tiles whose interior nodes hold reaction nodes, then the
+    // workgroups of the boundary kernel that do
+    const nsm::TileSet& ts = md->tiles;
+    for (int t = 0; t < ts.num_tiles; t++)
+      for (int s = ts.node_off[(size_t)t]; s < ts.node_off[(size_t)t + 1]; s++)
+        if ((ts.node_info[(size_t)s] & nsm::kInteriorNode) && md->reaction_mask[(size_t)ts.nodes[(size_t)s]]) {
+          tiles.push_back(t);
+          break;
+        }
+    for (size_t b = 0; b < ts.bnd_nodes.size(); b++)
+      if (md->reaction_mask[(size_t)ts.bnd_nodes[b]]) {
+        const int g = ts.num_tiles + (int)(b / nsm::kBoundaryNodesPerGroup);
+        if (tiles.empty() || tiles.back() != g) tiles.push_back(g);
+      }
+  }
   NSM_HIP(hipMemcpy(md->flags, code.data(), code.size(), hipMemcpyHostToDevice));
   free_dev(md->reaction_tiles);
   md->num_reaction_tiles = 0;
@@ -287,6 +320,18 @@ int check_status(nsm_model* md)
   return NSM_OK;
 }
 
+nsm::BlockTable<float> float_table(const nsm_model* md, int b0, int nb)
+{
+  nsm::BlockTable<float> bt{};
+  bt.count = nb;
+  for (int b = 0; b <= nb; b++) bt.off[b] = md->blk_off[b0 + b];
+  for (int b = 0; b < nb; b++) {
+    bt.K[b] = (float)md->K[b0 + b];
+    bt.G[b] = (float)md->G[b0 + b];
+  }
+  return bt;
+}
+
 // All blocks in one launch when they fit one block table, else groups of
 // kMaxLaunchBlocks consecutive blocks.
 int launch_elements(nsm_model* md)
@@ -307,14 +352,7 @@ int launch_elements(nsm_model* md)
       nsm::launch_element_fp64(bt, md->E, md->conn, md->X, md->u, (double*)md->Fip, (double*)md->sip,
                                (double*)md->slots, md->status, md->stream);
     } else {
-      nsm::BlockTable<float> bt{};
-      bt.count = nb;
-      for (int b = 0; b <= nb; b++) bt.off[b] = md->blk_off[b0 + b];
-      for (int b = 0; b < nb; b++) {
-        bt.K[b] = (float)md->K[b0 + b];
-        bt.G[b] = (float)md->G[b0 + b];
-      }
-      nsm::launch_element_mixed(bt, md->E, md->conn, md->X, md->u, (float*)md->Fip, (float*)md->sip,
+      nsm::launch_element_mixed(float_table(md, b0, nb), md->E, md->conn, md->X, md->u, (float*)md->Fip, (float*)md->sip,
                                 (float*)md->slots, md->status, md->stream);
     }
     NSM_LAUNCHED();
@@ -323,32 +361,133 @@ int launch_elements(nsm_model* md)
   return NSM_OK;
 }
 
-// Node kernel; the reaction sum of `step` (if a reaction set exists and the
-// step is recorded) is folded into it.
-int launch_node(nsm_model* md, nsm::StepTimes cur, bool output_bc, bool fuse, nsm::StepTimes nxt, bool write_a,
-                bool update_v, int64_t step)
+// Reaction sum of `step` (if a reaction set exists and the step is recorded):
+// the arguments the kernels fill their partials with.
+bool reaction_args(const nsm_model* md, int64_t step, nsm::ReactionArgs* rx)
+{
+  *rx                = nsm::ReactionArgs{-1, nullptr};
+  const bool want_rx = md->num_reaction_tiles > 0 && step >= 0 && step < md->reaction_cap;
+  if (want_rx) *rx = nsm::ReactionArgs{md->reaction_coord, md->reaction_partials};
+  return want_rx;
+}
+
+int launch_reaction(nsm_model* md, int64_t step)
 {
   size_t ti;
-  int    st = time_begin(md, NSM_KERNEL_NODE, &ti);
+  int    st = time_begin(md, NSM_KERNEL_REACTION, &ti);
   if (st) return st;
-  nsm::ReactionArgs rx{-1, nullptr};
-  const bool        want_rx = md->num_reaction_tiles > 0 && step >= 0 && step < md->reaction_cap;
-  if (want_rx) rx = nsm::ReactionArgs{md->reaction_coord, md->reaction_partials};
-  const nsm::NodeArrays na = md->node_arrays();
-  if (md->precision == NSM_FP64)
-    nsm::launch_node_update<double>(na, (const double*)md->slots, cur, output_bc, fuse, nxt, write_a, update_v, rx,
-                                    md->stream);
-  else
-    nsm::launch_node_update<float>(na, (const float*)md->slots, cur, output_bc, fuse, nxt, write_a, update_v, rx,
-                                   md->stream);
-  NSM_LAUNCHED();
-  if ((st = time_end(md, ti))) return st;
-  if (!want_rx) return NSM_OK;
-  if ((st = time_begin(md, NSM_KERNEL_REACTION, &ti))) return st;
   nsm::launch_reaction_finalize(md->reaction_partials, md->reaction_tiles, md->num_reaction_tiles,
                                 md->reaction_hist + step, md->stream);
   NSM_LAUNCHED();
   return time_end(md, ti);
+}
+
+// Node kernel of the slot path; the reaction partials of `step` are folded
+// into it.
+int launch_node(nsm_model* md, const nsm::StepArgs& sa, int64_t step)
+{
+  size_t ti;
+  int    st = time_begin(md, NSM_KERNEL_NODE, &ti);
+  if (st) return st;
+  nsm::ReactionArgs     rx;
+  const bool            want_rx = reaction_args(md, step, &rx);
+  const nsm::NodeArrays na      = md->node_arrays();
+  if (md->precision == NSM_FP64)
+    nsm::launch_node_update<double>(na, (const double*)md->slots, sa, rx, md->stream);
+  else
+    nsm::launch_node_update<float>(na, (const float*)md->slots, sa, rx, md->stream);
+  NSM_LAUNCHED();
+  if ((st = time_end(md, ti))) return st;
+  return want_rx ? launch_reaction(md, step) : NSM_OK;
+}
+
+// Tile-local step: element tiles (with the interior-node update), then the
+// boundary-node update, then the reaction sum.
+int launch_tiled(nsm_model* md, const nsm::StepArgs& sa, int64_t step)
+{
+  nsm::ReactionArgs     rx;
+  const bool            want_rx = reaction_args(md, step, &rx);
+  const nsm::NodeArrays na      = md->node_arrays();
+  int                   st      = NSM_OK;
+  for (int b0 = 0; b0 < md->num_blocks; b0 += nsm::kMaxLaunchBlocks) {
+    const int nb = std::min(nsm::kMaxLaunchBlocks, md->num_blocks - b0);
+    const int t0 = md->blk_tile_off[(size_t)b0], t1 = md->blk_tile_off[(size_t)(b0 + nb)];
+    size_t    ti;
+    if ((st = time_begin(md, NSM_KERNEL_ELEMENT, &ti))) return st;
+    nsm::launch_element_tiled(float_table(md, b0, nb), md->E, md->tile_args, t0, t1 - t0, md->X, na,
+                              (float*)md->Fip, (float*)md->sip, sa, rx, md->status, md->stream);
+    NSM_LAUNCHED();
+    if ((st = time_end(md, ti))) return st;
+  }
+  size_t ti;
+  if ((st = time_begin(md, NSM_KERNEL_NODE, &ti))) return st;
+  nsm::launch_boundary_update(md->tile_args, na, sa, rx, md->stream);
+  NSM_LAUNCHED();
+  if ((st = time_end(md, ti))) return st;
+  return want_rx ? launch_reaction(md, step) : NSM_OK;
+}
+
+// One step's force evaluation and node update (update_v = 0: f_int only).
+int launch_step(nsm_model* md, const nsm::StepArgs& sa, int64_t step)
+{
+  if (md->tiled) return launch_tiled(md, sa, step);
+  int st = launch_elements(md);
+  return st ? st : launch_node(md, sa, step);
+}
+
+// Device copies of the tile arrays.
+template <typename T>
+int upload(nsm_model* md, const std::vector<T>& h, const T** out)
+{
+  T*  p  = nullptr;
+  int st = dalloc(&p, std::max<size_t>(h.size(), 1), &md->device_bytes);
+  if (st) return st;
+  md->tile_bufs.push_back((void*)p);
+  if (!h.empty()) NSM_HIP(hipMemcpy(p, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice));
+  *out = p;
+  return NSM_OK;
+}
+
+int setup_tiles(nsm_model* md)
+{
+  const nsm::TileSet& ts = md->tiles;
+  nsm::TileArgs&      ta = md->tile_args;
+  // the element kernel runs one thread per tile element and stages at most
+  // two tile nodes per thread
+  if (ts.tile_elems != 128 && ts.tile_elems != 256) return fail(NSM_ERR_ARG, "unsupported tile size");
+  for (int t = 0; t < ts.num_tiles; t++)
+    if (ts.elem_off[(size_t)t + 1] - ts.elem_off[(size_t)t] > ts.tile_elems ||
+        ts.node_off[(size_t)t + 1] - ts.node_off[(size_t)t] > 2 * ts.tile_elems)
+      return fail(NSM_ERR_ARG, "element tile %d exceeds the kernel's tile size", t);
+  ta.tile_elems          = ts.tile_elems;
+  ta.num_tiles           = ts.num_tiles;
+  int cus                = 0;
+  NSM_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, md->device));
+  ta.resident_groups = std::max(cus, 1) * std::max(nsm::element_tiled_groups_per_cu(ts.tile_elems), 1);
+  ta.num_bnd             = (int)ts.bnd_nodes.size();
+  int st                 = 0;
+  if ((st = upload(md, ts.elem_off, &ta.elem_off)) || (st = upload(md, ts.node_off, &ta.node_off)) ||
+      (st = upload(md, ts.nodes, &ta.nodes)) || (st = upload(md, ts.node_info, &ta.node_info)) ||
+      (st = upload(md, ts.lconn, &ta.lconn)) || (st = upload(md, ts.color, &ta.color)) ||
+      (st = upload(md, std::vector<int32_t>(ts.num_colors.begin(), ts.num_colors.end()), &ta.num_colors)) || (st = upload(md, ts.bnd_nodes, &ta.bnd_nodes)) ||
+      (st = upload(md, ts.bnd_ptr, &ta.bnd_ptr)) || (st = upload(md, ts.bnd_ent, &ta.bnd_ent)))
+    return st;
+  double* partials = nullptr;
+  if ((st = dalloc(&partials, std::max<size_t>(3 * ts.nodes.size(), 1), &md->device_bytes))) return st;
+  md->tile_bufs.push_back((void*)partials);
+  ta.partials = partials;
+  md->blk_tile_off.assign((size_t)md->num_blocks + 1, 0);
+  for (int b = 0; b <= md->num_blocks; b++)
+    md->blk_tile_off[(size_t)b] =
+        (int)(std::lower_bound(ts.elem_off.begin(), ts.elem_off.end(), (int32_t)md->blk_off[(size_t)b]) -
+              ts.elem_off.begin());
+  return NSM_OK;
+}
+
+// original element (conn order) stored at device element position e
+inline int64_t element_of(const nsm_model* md, int64_t e)
+{
+  return md->tiled ? md->tiles.perm[(size_t)e] : e;
 }
 
 nsm::StepTimes step_times(double* time_current, double dt_user)
@@ -515,19 +654,39 @@ int nsm_model_create(int64_t num_nodes, const double* reference_coordinates, int
   if (hipStreamCreateWithFlags(&md->stream, hipStreamNonBlocking) != hipSuccess)
     return cleanup(fail(NSM_ERR_HIP, "hipStreamCreate failed"));
 
+  // Mixed precision assembles forces tile-locally unless the mesh cannot be
+  // tiled (an element tile needing more than kMaxColors colours) or
+  // NSM_MIXED_ASSEMBLY=slots asks for the element-slot + node-gather path.
+  if (precision == NSM_MIXED && E > 0) {
+    const char* mode = std::getenv("NSM_MIXED_ASSEMBLY");
+    if (!(mode && std::strcmp(mode, "slots") == 0)) {
+      // NSM_TILE_ELEMS=128 selects 128-element tiles (two waves per workgroup)
+      const char* te         = std::getenv("NSM_TILE_ELEMS");
+      const int   tile_elems = (te && std::strcmp(te, "128") == 0) ? 128 : nsm::kTileElems;
+      std::string err;
+      md->tiled = nsm::build_tiles(num_nodes, reference_coordinates, conn, off, tile_elems, md->tiles, &err);
+      if (!md->tiled) md->tiles = nsm::TileSet();
+    }
+  }
+
   const size_t N3 = 3 * (size_t)num_nodes;
   const size_t ipw = precision == NSM_FP64 ? sizeof(double) : sizeof(float);
   size_t&      tot = md->device_bytes;
   int          st  = 0;
+  const size_t num_slots    = md->tiled ? 0 : (size_t)(24 * E);
+  const size_t num_partials = md->tiled ? (size_t)md->tiles.num_tiles +
+                                              nsm::boundary_grid((int)md->tiles.bnd_nodes.size())
+                                        : (size_t)nsm::node_grid(num_nodes);
   if ((st = dalloc(&md->X, N3, &tot)) || (st = dalloc(&md->u, N3, &tot)) || (st = dalloc(&md->v, N3, &tot)) ||
       (st = dalloc(&md->a, N3, &tot)) || (st = dalloc(&md->f, N3, &tot)) ||
       (st = dalloc(&md->m, (size_t)num_nodes, &tot)) || (st = dalloc(&md->conn, (size_t)(8 * E), &tot)) ||
       (st = dalloc(&md->node_ptr, ptr.size(), &tot)) || (st = dalloc(&md->node_ent, ent.size(), &tot)) ||
       (st = dalloc((char**)&md->Fip, (size_t)(72 * E) * ipw, &tot)) ||
       (st = dalloc((char**)&md->sip, (size_t)(48 * E) * ipw, &tot)) ||
-      (st = dalloc((char**)&md->slots, (size_t)(24 * E) * ipw, &tot)) ||
+      (st = dalloc((char**)&md->slots, num_slots * ipw, &tot)) ||
       (st = dalloc(&md->status, 1, &tot)) || (st = dalloc(&md->flags, (size_t)num_nodes, &tot)) ||
-      (st = dalloc(&md->reaction_partials, (size_t)nsm::node_grid(num_nodes), &tot)))
+      (st = dalloc(&md->reaction_partials, std::max<size_t>(num_partials, 1), &tot)) ||
+      (md->tiled && (st = setup_tiles(md))))
     return cleanup(st);
 
   hipStream_t s = md->stream;
@@ -594,6 +753,7 @@ int nsm_model_destroy(nsm_model* md)
   free_dev(md->reaction_hist);
   free_dev(md->reaction_partials);
   free_dev(md->reaction_tiles);
+  for (void* p : md->tile_bufs) (void)hipFree(p);
   for (auto& p : md->event_pool) {
     (void)hipEventDestroy(p.start);
     (void)hipEventDestroy(p.stop);
@@ -712,9 +872,10 @@ int nsm_model_compute_internal_force(nsm_model* md, double time_previous, double
   (void)is_output_step;
   if (!md) return fail(NSM_ERR_ARG, "null model");
   int st = ensure_device(md);
-  if (st || (st = launch_elements(md))) return st;
-  nsm::StepTimes none{0.0, 0.0, 0.0, 0.0};
-  if ((st = launch_node(md, none, false, false, none, false, false, -1))) return st;
+  if (st) return st;
+  const nsm::StepTimes none{0.0, 0.0, 0.0, 0.0};
+  const nsm::StepArgs  sa{none, none, 0, 0, 0, 0};
+  if ((st = launch_step(md, sa, -1))) return st;
   return check_status(md);
 }
 
@@ -785,14 +946,12 @@ int nsm_model_explicit_steps(nsm_model* md, int64_t step_begin, int64_t step_end
   nsm::launch_prekick(md->node_arrays(), cur, md->stream);
   NSM_LAUNCHED();
   for (int64_t step = step_begin; step < step_end; step++) {
-    if ((st = launch_elements(md))) return st;
     const bool     last = step + 1 == step_end;
     double         tn   = t;
     nsm::StepTimes nxt  = cur;
     if (!last) nxt = step_times(&tn, dt_user);
-    if ((st = launch_node(md, cur, is_output(step, num_load_steps, output_frequency), !last, nxt, last, true,
-                          step)))
-      return st;
+    const nsm::StepArgs sa{cur, nxt, (int)is_output(step, num_load_steps, output_frequency), (int)!last, (int)last, 1};
+    if ((st = launch_step(md, sa, step))) return st;
     if (!last) {
       t   = tn;
       cur = nxt;
@@ -838,7 +997,7 @@ int nsm_model_get_field(nsm_model* md, int field, double* out)
       for (size_t c = 0; c < (size_t)nc; c++)
         for (size_t q = 0; q < 8; q++) {
           const double* src_row = tmp.data() + (c * 8 + q) * E;
-          for (size_t e = 0; e < E; e++) out[(e * 8 + q) * nc + c] = src_row[e];
+          for (size_t e = 0; e < E; e++) out[((size_t)element_of(md, (int64_t)e) * 8 + q) * nc + c] = src_row[e];
         }
       return NSM_OK;
     }
@@ -932,6 +1091,24 @@ int nsm_model_kernel_time(nsm_model* md, int kernel, double* avg_ms, int64_t* la
   }
   *avg_ms = n ? sum / (double)n : 0.0;
   if (launches) *launches = n;
+  return NSM_OK;
+}
+
+int nsm_model_get_element_order(nsm_model* md, int64_t* out)
+{
+  if (!md || (md->E > 0 && !out)) return fail(NSM_ERR_ARG, "null argument");
+  for (int64_t e = 0; e < md->E; e++) out[e] = element_of(md, e);
+  return NSM_OK;
+}
+
+int nsm_model_assembly_info(nsm_model* md, int* tiled, int64_t* num_tiles, int64_t* interior_nodes,
+                            int64_t* boundary_nodes)
+{
+  if (!md) return fail(NSM_ERR_ARG, "null model");
+  if (tiled) *tiled = md->tiled ? 1 : 0;
+  if (num_tiles) *num_tiles = md->tiles.num_tiles;
+  if (interior_nodes) *interior_nodes = md->tiles.num_interior;
+  if (boundary_nodes) *boundary_nodes = md->tiled ? (int64_t)md->tiles.bnd_nodes.size() : md->N;
   return NSM_OK;
 }
 

@@ -8,51 +8,12 @@
 // adds element contributions in ascending element order starting from 0.0,
 // which is the serial scatter order of src/nimble_block.cc:374-383 and
 // src/nimble_model_data.cc:636-659.
-#include "nsm_kernels.h"
+#include "nsm_node_step.h"
 
 namespace nsm {
 namespace {
 
 constexpr int kNodeThreads = 256;
-
-// ApplyKinematicBC for the explicit scheme
-// (src/nimble_boundary_condition_manager.h:136-217), resolved per node.
-__device__ __forceinline__ void apply_bc(const NodeArrays& na, int code, double v[3], const double u[3],
-                                         double delta_t)
-{
-  if (code == 0) return;
-  const BcEntry& b = na.bc_table[code - 1];
-#pragma unroll
-  for (int c = 0; c < 3; c++) {
-    if (b.kind[c] == 1)
-      v[c] = b.value[c];
-    else if (b.kind[c] == 2 && delta_t > 0.0)
-      v[c] = (b.value[c] - u[c]) / delta_t;
-  }
-}
-
-__device__ __forceinline__ void load3(const double* p, int64_t i, double r[3])
-{
-  r[0] = p[3 * i];
-  r[1] = p[3 * i + 1];
-  r[2] = p[3 * i + 2];
-}
-
-__device__ __forceinline__ void store3(double* p, int64_t i, const double r[3])
-{
-  p[3 * i]     = r[0];
-  p[3 * i + 1] = r[1];
-  p[3 * i + 2] = r[2];
-}
-
-// kick: v += (h * 1.0) * a  (also the drift u += (dt * 1.0) * v)
-__device__ __forceinline__ void kick(double v[3], const double a[3], double h)
-{
-  const double prod = h * 1.0;
-  v[0] += prod * a[0];
-  v[1] += prod * a[1];
-  v[2] += prod * a[2];
-}
 
 __global__ __launch_bounds__(kNodeThreads) void k_apply_bc(NodeArrays na, StepTimes t)
 {
@@ -63,7 +24,7 @@ __global__ __launch_bounds__(kNodeThreads) void k_apply_bc(NodeArrays na, StepTi
   double u[3], v[3];
   load3(na.u, i, u);
   load3(na.v, i, v);
-  apply_bc(na, code, v, u, t.t_cur - t.t_prev);
+  apply_bc(na.bc_table, code, v, u, t.t_cur - t.t_prev);
   store3(na.v, i, v);
 }
 
@@ -79,9 +40,9 @@ __global__ __launch_bounds__(kNodeThreads) void k_prekick(NodeArrays na, StepTim
   load3(na.a, i, a);
   const double delta_t = t.t_cur - t.t_prev;
   kick(v, a, t.half_dt);
-  apply_bc(na, code, v, u, delta_t);
+  apply_bc(na.bc_table, code, v, u, delta_t);
   kick(u, v, t.dt);
-  apply_bc(na, code, v, u, delta_t);
+  apply_bc(na.bc_table, code, v, u, delta_t);
   store3(na.u, i, u);
   store3(na.v, i, v);
 }
@@ -90,10 +51,9 @@ __global__ __launch_bounds__(kNodeThreads) void k_prekick(NodeArrays na, StepTim
 // (src/integrators/explicit_time_integrator.cc:250-270), optionally fused
 // with the first half of step n+1, plus per-tile partial reaction sums over
 // flagged nodes.
-template <typename SlotT, bool kFuse>
+template <typename SlotT>
 __global__ __launch_bounds__(kNodeThreads) void k_node_update(NodeArrays na, const SlotT* __restrict__ slots,
-                                                              StepTimes cur, int output_bc, StepTimes nxt,
-                                                              int write_a, int update_v, ReactionArgs rx)
+                                                              StepArgs sa, ReactionArgs rx)
 {
   const unsigned tile    = xcd_tile(blockIdx.x, gridDim.x);
   const int64_t  i       = (int64_t)tile * kNodeThreads + threadIdx.x;
@@ -126,34 +86,14 @@ __global__ __launch_bounds__(kNodeThreads) void k_node_update(NodeArrays na, con
           f[2] += (double)s2[k];
         }
     }
-    store3(na.f, i, f);
     const int flags = na.flags[i];
     if (rx.coord >= 0 && (flags & kReactionFlag)) {
       member  = 1;
       contrib = f[rx.coord];
     }
-    if (update_v) {
-      const int    code     = flags & 0x7f;
-      const double oneOverM = 1.0 / na.mass[i];
-      double       a[3], u[3], v[3];
-      a[0] = oneOverM * (f[0] + 0.0);
-      a[1] = oneOverM * (f[1] + 0.0);
-      a[2] = oneOverM * (f[2] + 0.0);
-      load3(na.u, i, u);
-      load3(na.v, i, v);
-      kick(v, a, cur.half_dt);
-      if (output_bc) apply_bc(na, code, v, u, cur.t_cur - cur.t_prev);
-      if (kFuse) {
-        const double delta_t = nxt.t_cur - nxt.t_prev;
-        kick(v, a, nxt.half_dt);
-        apply_bc(na, code, v, u, delta_t);
-        kick(u, v, nxt.dt);
-        apply_bc(na, code, v, u, delta_t);
-        store3(na.u, i, u);
-      }
-      store3(na.v, i, v);
-      if (write_a) store3(na.a, i, a);
-    }
+    double u[3];
+    if (sa.update_v) load3(na.u, i, u);
+    node_step(na, i, flags, f, u, sa);
   }
   if (rx.coord < 0) return;  // uniform across the grid
 
@@ -214,22 +154,16 @@ void launch_prekick(const NodeArrays& na, StepTimes t, hipStream_t s)
 }
 
 template <typename SlotT>
-void launch_node_update(const NodeArrays& na, const SlotT* slots, StepTimes cur, bool output_bc, bool fuse_next,
-                        StepTimes next, bool write_a, bool update_v, ReactionArgs rx, hipStream_t s)
+void launch_node_update(const NodeArrays& na, const SlotT* slots, const StepArgs& sa, ReactionArgs rx, hipStream_t s)
 {
   if (na.num_nodes <= 0) return;
-  if (fuse_next)
-    hipLaunchKernelGGL((k_node_update<SlotT, true>), dim3(node_grid(na.num_nodes)), dim3(kNodeThreads), 0, s, na,
-                       slots, cur, (int)output_bc, next, (int)write_a, (int)update_v, rx);
-  else
-    hipLaunchKernelGGL((k_node_update<SlotT, false>), dim3(node_grid(na.num_nodes)), dim3(kNodeThreads), 0, s, na,
-                       slots, cur, (int)output_bc, next, (int)write_a, (int)update_v, rx);
+  hipLaunchKernelGGL((k_node_update<SlotT>), dim3(node_grid(na.num_nodes)), dim3(kNodeThreads), 0, s, na, slots, sa,
+                     rx);
 }
 
-template void launch_node_update<double>(const NodeArrays&, const double*, StepTimes, bool, bool, StepTimes, bool,
-                                         bool, ReactionArgs, hipStream_t);
-template void launch_node_update<float>(const NodeArrays&, const float*, StepTimes, bool, bool, StepTimes, bool, bool,
-                                        ReactionArgs, hipStream_t);
+template void launch_node_update<double>(const NodeArrays&, const double*, const StepArgs&, ReactionArgs,
+                                         hipStream_t);
+template void launch_node_update<float>(const NodeArrays&, const float*, const StepArgs&, ReactionArgs, hipStream_t);
 
 void launch_reaction_finalize(const double* partials, const int* tiles, int num_tiles, double* out, hipStream_t s)
 {

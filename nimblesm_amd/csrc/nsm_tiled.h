@@ -1,0 +1,46 @@
+// nsm_tiled.h -- launchers of the tile-local mixed-precision step (internal).
+//
+// Device arrays of a TileSet (nsm_tiles.h) plus the FP64 partial force of
+// every tile-node slot that is not interior to its tile.
+#pragma once
+
+#include "nsm_kernels.h"
+
+namespace nsm {
+
+struct TileArgs {
+  int             tile_elems;       // 256 or 128: threads per workgroup, nodes per tile <= 2*tile_elems
+  int             resident_groups;  // element-kernel workgroups resident on the device at once
+  int             num_tiles;
+  int             num_bnd;
+  const int32_t*  elem_off;    // [T+1]
+  const int32_t*  node_off;    // [T+1]
+  const int32_t*  nodes;       // [slots]
+  const uint8_t*  node_info;   // [slots]
+  const uint16_t* lconn;       // [E][8], internal element order
+  const uint8_t*  color;       // [E]
+  const int32_t*  num_colors;  // [T]
+  const int32_t*  bnd_nodes;   // [B]
+  const int32_t*  bnd_ptr;     // [B+1]
+  const int32_t*  bnd_ent;     // partial slots, ascending tile per node
+  double*         partials;    // [slots][3]
+};
+
+// boundary nodes per workgroup of the boundary kernel (one per thread)
+constexpr int kBoundaryNodesPerGroup = 256;
+unsigned      boundary_grid(int num_bnd);
+
+// Tiles [t_begin, t_begin + t_count) (all inside the blocks of bt): element
+// arithmetic, force assembly in LDS, the step of the tile-interior nodes,
+// partials of the others.  Reaction partials go to rx.partials[tile].
+void launch_element_tiled(const BlockTable<float>& bt, int64_t E_total, const TileArgs& ta, int t_begin, int t_count,
+                          const double* X, const NodeArrays& na, float* Fip, float* sip, const StepArgs& sa,
+                          ReactionArgs rx, int* status, hipStream_t s);
+// Element-kernel workgroups one CU holds at once (occupancy).
+int element_tiled_groups_per_cu(int tile_elems);
+// Step of the remaining nodes from their partials (ascending tile order).
+// Reaction partials go to rx.partials[num_tiles + workgroup].
+void launch_boundary_update(const TileArgs& ta, const NodeArrays& na, const StepArgs& sa, ReactionArgs rx,
+                            hipStream_t s);
+
+}  // namespace nsm

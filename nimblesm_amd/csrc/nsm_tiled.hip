@@ -1,0 +1,356 @@
+// nsm_tiled.hip -- mixed-precision step with tile-local force assembly.
+//
+// k_element_tiled: one workgroup per element tile (nsm_tiles.h), one thread
+// per element.  The tile's nodes (X, u in FP64) are staged in LDS; each
+// thread runs the mixed-precision element arithmetic (nsm_mixed_math.h),
+// writes F and sigma, and its 24 node forces are added into FP64 LDS
+// accumulators colour by colour (fixed order).  Tile-interior nodes then take
+// their whole central-difference update here; every other node gets this
+// tile's partial sum.  k_boundary_update adds the partials of the remaining
+// nodes in ascending tile order and updates them.  Each kernel also leaves one
+// reaction partial per workgroup.
+//
+// The loads a tile needs are issued before its element arithmetic (one tile
+// ahead for X, u and element data, two for node ids; at the tile's start for
+// the flags, mass and v of its interior nodes) and the kinematic-condition
+// table sits in LDS: the F and sigma stores issued during the arithmetic
+// count in the same in-order vector-memory counter as loads, so a load
+// issued after them would wait until all of them had completed.
+#include "nsm_mixed_math.h"
+#include "nsm_node_step.h"
+#include "nsm_tiled.h"
+#include "nsm_tiles.h"
+
+#include <algorithm>
+
+namespace nsm {
+namespace {
+
+// Sum over the workgroup in a fixed order (butterfly inside each 64-lane
+// wave, then the wave sums in ascending wave order); valid in thread 0.
+template <int kThreads>
+__device__ __forceinline__ double block_sum(double v, double* red)
+{
+#pragma unroll
+  for (int m = 32; m > 0; m >>= 1) v += __shfl_xor(v, m, 64);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  double s = 0.0;
+  if (threadIdx.x == 0)
+#pragma unroll
+    for (int w = 0; w < kThreads / 64; w++) s += red[w];
+  return s;
+}
+
+// Tile indices are uniform; readfirstlane keeps them in scalar registers, and
+// the per-tile tables (read-only for the whole launch) are read through the
+// constant address space, so they come in by scalar loads, which do not
+// count in the vector-memory counter.
+__device__ __forceinline__ int uniform(int t) { return __builtin_amdgcn_readfirstlane(t); }
+
+template <typename T>
+__device__ __forceinline__ T tile_table(const T* p, int t)
+{
+  return ((const __attribute__((address_space(4))) T*)p)[t];
+}
+
+// Node ids of the (at most two) tile-node slots a thread stages.  A thread
+// past the tile's last slot gets that last slot's node (so every load stays
+// unconditional) with kAbsent set.
+constexpr int kAbsent = 0x100;
+struct SlotIds {
+  int gid[2];
+  int info[2];  // node_info | kAbsent
+};
+
+template <int kThreads>
+__device__ __forceinline__ SlotIds load_ids(const TileArgs& ta, int t)
+{
+  t            = uniform(t);
+  const int n0 = tile_table(ta.node_off, t), nn = tile_table(ta.node_off, t + 1) - n0;
+  SlotIds   r;
+#pragma unroll
+  for (int k = 0; k < 2; k++) {
+    const int ln = (int)threadIdx.x + k * kThreads;
+    const int sl = n0 + min(ln, nn - 1);
+    r.gid[k]     = ta.nodes[sl];
+    r.info[k]    = ta.node_info[sl] | (ln < nn ? 0 : kAbsent);
+  }
+  return r;
+}
+
+// Element data of one thread: local connectivity (8 x uint16) and colour
+// byte.  A thread past the tile's last element takes that last element (it
+// recomputes it and rewrites identical F and sigma) with colour 0xff, so it
+// adds no force.
+struct ElemData {
+  uint4 lq;
+  int   cinfo;
+};
+
+template <int kThreads>
+__device__ __forceinline__ ElemData load_elem(const TileArgs& ta, int t)
+{
+  t               = uniform(t);
+  const int e0    = tile_table(ta.elem_off, t), ne = tile_table(ta.elem_off, t + 1) - e0;
+  const int64_t e = e0 + (int64_t)min((int)threadIdx.x, ne - 1);
+  ElemData      d;
+  d.lq    = *reinterpret_cast<const uint4*>(ta.lconn + 8 * e);
+  d.cinfo = (int)threadIdx.x < ne ? (int)ta.color[e] : 0xff;
+  return d;
+}
+
+// Staging of one tile: its nodes' X and u into LDS, zeroed accumulators,
+// and the loads of its interior nodes' flags (and v, m when the step updates
+// velocities).  Values of non-interior slots are left unset: they are never
+// read, and no merge of a default forces an early wait.
+struct NodeState {
+  int    flg[2];
+  double v[2][3], m[2];
+};
+
+template <int kThreads>
+__device__ __forceinline__ void stage(const SlotIds& ids, const double (&xr)[2][3], const double (&ur)[2][3],
+                                      const NodeArrays& na, const StepArgs& sa, double (*sX)[3], double (*su)[3],
+                                      double (*acc)[3], NodeState& ns)
+{
+#pragma unroll
+  for (int k = 0; k < 2; k++) {
+    const int ln = (int)threadIdx.x + k * kThreads;
+    if (!(ids.info[k] & kAbsent)) {
+#pragma unroll
+      for (int c = 0; c < 3; c++) {
+        sX[ln][c]  = xr[k][c];
+        su[ln][c]  = ur[k][c];
+        acc[ln][c] = 0.0;
+      }
+    }
+    if ((ids.info[k] & (kInteriorNode | kAbsent)) == kInteriorNode) {
+      ns.flg[k] = na.flags[ids.gid[k]];
+      if (sa.update_v) {
+        load3(na.v, ids.gid[k], ns.v[k]);
+        ns.m[k] = na.mass[ids.gid[k]];
+      }
+    }
+  }
+}
+
+// Persistent workgroups, each walking a contiguous range of tiles (the
+// workgroups of one XCD cover one contiguous range of the Morton-ordered
+// tiles).  kThreads elements and at most 2*kThreads nodes per tile.  While
+// tile t computes, X, u and element data of tile t+1 and node ids of tile t+2
+// are in flight; tile t+1 is staged into LDS at the end of the iteration.
+// Inside the loop every vector load and every F / sigma store is executed on
+// every path (indices clamped), so at least the 120 stores of the element
+// arithmetic always separate a load from its first use and the waits the
+// compiler places never wait for those stores.
+template <int kThreads>
+__global__ __launch_bounds__(kThreads) void k_element_tiled(const BlockTable<float> bt, int64_t E_total,
+                                                            TileArgs ta, int t_begin, int t_count,
+                                                            const double* __restrict__ X, NodeArrays na,
+                                                            float* __restrict__ Fip, float* __restrict__ sip,
+                                                            StepArgs sa, ReactionArgs rx, int* __restrict__ status)
+{
+  constexpr int kNodes = 2 * kThreads;
+  __shared__ double  sX[kNodes][3];
+  __shared__ double  su[kNodes][3];
+  __shared__ double  acc[kNodes][3];
+  __shared__ double  red[kThreads / 64];
+  __shared__ BcEntry sbc[kMaxBcEntries];
+  __shared__ int     bad;
+
+  const int w  = (int)xcd_tile(blockIdx.x, gridDim.x);
+  const int tb = uniform(t_begin + (int)((int64_t)t_count * w / gridDim.x));
+  const int te = uniform(t_begin + (int)((int64_t)t_count * (w + 1) / gridDim.x));
+  if (tb >= te) return;  // uniform across the workgroup
+  const int tid = (int)threadIdx.x;
+
+  // ---- prologue: stage tile tb, load tile tb+1's node ids
+  if (tid == 0) bad = 0;
+  if (sa.update_v)
+    for (int b = tid; b < na.num_bc; b += kThreads) sbc[b] = na.bc_table[b];
+  SlotIds ids      = load_ids<kThreads>(ta, tb);
+  SlotIds ids_next = load_ids<kThreads>(ta, min(tb + 1, te - 1));
+  double  xr[2][3], ur[2][3];
+#pragma unroll
+  for (int k = 0; k < 2; k++) {
+    load3(X, ids.gid[k], xr[k]);
+    load3(na.u, ids.gid[k], ur[k]);
+  }
+  ElemData  cur = load_elem<kThreads>(ta, tb);
+  NodeState ns;
+  stage<kThreads>(ids, xr, ur, na, sa, sX, su, acc, ns);
+  // every prologue load complete (vmcnt(0); lgkmcnt and expcnt at their
+  // maximum, gfx9 encoding): the loop's waits then depend on its body only
+  __builtin_amdgcn_s_waitcnt(0x0F70);
+
+  for (int tl = tb; tl < te; tl++) {
+    const int t    = uniform(tl);
+    const int e0   = tile_table(ta.elem_off, t);
+    const int n0   = tile_table(ta.node_off, t);
+    const int ncol = tile_table(ta.num_colors, t);
+    __syncthreads();  // tile t staged
+
+    // ---- prefetch (clamped to the last tile): tile t+1's nodes and
+    // elements, tile t+2's node ids
+#pragma unroll
+    for (int k = 0; k < 2; k++) {
+      load3(X, ids_next.gid[k], xr[k]);
+      load3(na.u, ids_next.gid[k], ur[k]);
+    }
+    const ElemData el_next   = load_elem<kThreads>(ta, min(t + 1, te - 1));
+    const SlotIds  ids_after = load_ids<kThreads>(ta, min(t + 2, te - 1));
+
+    // ---- element arithmetic (every thread; see load_elem)
+    const int64_t e = e0 + min(tid, tile_table(ta.elem_off, t + 1) - e0 - 1);
+    float         f[8][3];
+    uint16_t      lc[8];
+    lc[0] = (uint16_t)(cur.lq.x & 0xffff);
+    lc[1] = (uint16_t)(cur.lq.x >> 16);
+    lc[2] = (uint16_t)(cur.lq.y & 0xffff);
+    lc[3] = (uint16_t)(cur.lq.y >> 16);
+    lc[4] = (uint16_t)(cur.lq.z & 0xffff);
+    lc[5] = (uint16_t)(cur.lq.z >> 16);
+    lc[6] = (uint16_t)(cur.lq.w & 0xffff);
+    lc[7] = (uint16_t)(cur.lq.w >> 16);
+    {
+      double Xe[8][3], ue[8][3];
+#pragma unroll
+      for (int j = 0; j < 8; j++)
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+          Xe[j][c] = sX[lc[j]][c];
+          ue[j][c] = su[lc[j]][c];
+        }
+      const int b     = uniform(block_of(bt, (int64_t)e0));  // tiles never straddle blocks
+      auto      store = [&](int q, const float* Fq, const float* Sq) {
+#pragma unroll
+        for (int c = 0; c < 9; c++) Fip[(int64_t)(c * 8 + q) * E_total + e] = Fq[c];
+#pragma unroll
+        for (int c = 0; c < 6; c++) sip[(int64_t)(c * 8 + q) * E_total + e] = Sq[c];
+      };
+      if (!mixed::element(Xe, ue, bt.K[b], bt.G[b], store, f)) bad = 1;
+    }
+
+    // ---- colour-ordered assembly: the elements of one colour touch distinct
+    // nodes.  An element's 8 nodes are distinct unless it is flagged
+    // kRepeatedNode, so its 24 updates are done as one batch of reads, adds
+    // and writes; a flagged element adds node by node.
+    const int  col      = cur.cinfo == 0xff ? -1 : (cur.cinfo & kColorMask);
+    const bool repeated = cur.cinfo != 0xff && (cur.cinfo & kRepeatedNode) != 0;
+    for (int c = 0; c < ncol; c++) {
+      if (col == c) {
+        if (!repeated) {
+          double a[8][3];
+#pragma unroll
+          for (int j = 0; j < 8; j++)
+#pragma unroll
+            for (int k = 0; k < 3; k++) a[j][k] = acc[lc[j]][k];
+#pragma unroll
+          for (int j = 0; j < 8; j++)
+#pragma unroll
+            for (int k = 0; k < 3; k++) acc[lc[j]][k] = a[j][k] + (double)f[j][k];
+        } else {
+          for (int j = 0; j < 8; j++)
+            for (int k = 0; k < 3; k++) acc[lc[j]][k] += (double)f[j][k];
+        }
+      }
+      __syncthreads();
+    }
+
+    // ---- interior nodes: complete step; others: this tile's partial
+    double rxs = 0.0;
+#pragma unroll
+    for (int k = 0; k < 2; k++) {
+      const int ln = tid + k * kThreads;
+      if (ids.info[k] & kAbsent) continue;
+      double fn[3] = {acc[ln][0], acc[ln][1], acc[ln][2]};
+      if (ids.info[k] & kInteriorNode) {
+        if (rx.coord >= 0 && (ns.flg[k] & kReactionFlag)) rxs += fn[rx.coord];
+        double un[3] = {su[ln][0], su[ln][1], su[ln][2]};
+        node_update(na, sbc, ids.gid[k], ns.flg[k], fn, un, ns.v[k], ns.m[k], sa);
+      } else {
+        store3(ta.partials, (int64_t)(n0 + ln), fn);
+      }
+    }
+    if (rx.coord >= 0) {  // uniform across the grid
+      const double s = block_sum<kThreads>(rxs, red);
+      if (threadIdx.x == 0) rx.partials[t] = s;
+    }
+    __syncthreads();  // tile t's LDS reads done
+
+    // ---- stage tile t+1
+    ids      = ids_next;
+    ids_next = ids_after;
+    cur      = el_next;
+    if (t + 1 < te) stage<kThreads>(ids, xr, ur, na, sa, sX, su, acc, ns);
+  }
+  if (tid == 0 && bad) atomicOr(status, 1);
+}
+
+constexpr int kBndThreads = kBoundaryNodesPerGroup;
+
+__global__ __launch_bounds__(kBndThreads) void k_boundary_update(TileArgs ta, NodeArrays na, StepArgs sa,
+                                                                 ReactionArgs rx)
+{
+  __shared__ double red[kBndThreads / 64];
+  const unsigned    g   = xcd_tile(blockIdx.x, gridDim.x);
+  const int         bi  = (int)g * kBndThreads + threadIdx.x;
+  double            rxs = 0.0;
+  if (bi < ta.num_bnd) {
+    const int64_t i  = ta.bnd_nodes[bi];
+    const int     p0 = ta.bnd_ptr[bi], p1 = ta.bnd_ptr[bi + 1];
+    double        f[3] = {0.0, 0.0, 0.0};
+    for (int p = p0; p < p1; p++) {
+      const int64_t s = ta.bnd_ent[p];
+      f[0] += ta.partials[3 * s];
+      f[1] += ta.partials[3 * s + 1];
+      f[2] += ta.partials[3 * s + 2];
+    }
+    const int flags = na.flags[i];
+    if (rx.coord >= 0 && (flags & kReactionFlag)) rxs = f[rx.coord];
+    double u[3];
+    if (sa.update_v) load3(na.u, i, u);
+    node_step(na, i, flags, f, u, sa);
+  }
+  if (rx.coord < 0) return;
+  const double s = block_sum<kBndThreads>(rxs, red);
+  if (threadIdx.x == 0) rx.partials[ta.num_tiles + g] = s;
+}
+
+}  // namespace
+
+unsigned boundary_grid(int num_bnd) { return (unsigned)((num_bnd + kBndThreads - 1) / kBndThreads); }
+
+void launch_element_tiled(const BlockTable<float>& bt, int64_t E_total, const TileArgs& ta, int t_begin, int t_count,
+                          const double* X, const NodeArrays& na, float* Fip, float* sip, const StepArgs& sa,
+                          ReactionArgs rx, int* status, hipStream_t s)
+{
+  if (t_count <= 0) return;
+  const unsigned grid = (unsigned)std::min(t_count, std::max(ta.resident_groups, 1));
+  if (ta.tile_elems == 128)
+    hipLaunchKernelGGL(k_element_tiled<128>, dim3(grid), dim3(128), 0, s, bt, E_total, ta, t_begin, t_count, X, na,
+                       Fip, sip, sa, rx, status);
+  else
+    hipLaunchKernelGGL(k_element_tiled<256>, dim3(grid), dim3(256), 0, s, bt, E_total, ta, t_begin, t_count, X, na,
+                       Fip, sip, sa, rx, status);
+}
+
+int element_tiled_groups_per_cu(int tile_elems)
+{
+  int n = 0;
+  if (tile_elems == 128)
+    (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_element_tiled<128>, 128, 0);
+  else
+    (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_element_tiled<256>, 256, 0);
+  return n;
+}
+
+void launch_boundary_update(const TileArgs& ta, const NodeArrays& na, const StepArgs& sa, ReactionArgs rx,
+                            hipStream_t s)
+{
+  if (ta.num_bnd <= 0) return;
+  hipLaunchKernelGGL(k_boundary_update, dim3(boundary_grid(ta.num_bnd)), dim3(kBndThreads), 0, s, ta, na, sa, rx);
+}
+
+}  // namespace nsm

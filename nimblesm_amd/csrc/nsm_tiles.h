@@ -1,0 +1,51 @@
+// nsm_tiles.h -- element tiles with tile-local force assembly (mixed path).
+//
+// Elements of each block are put in Morton order of their centroids and cut
+// into tiles of at most tile_elems elements (256 by default, or 128) touching
+// at most 2*tile_elems distinct nodes.  A node whose incident elements all lie in one tile is
+// "interior" to it: that tile sums its force completely and performs its
+// central-difference update in the element kernel.  Every other node gets one
+// partial sum per tile that touches it; the boundary kernel adds those
+// partials in ascending tile order.  Inside a tile, element forces are summed
+// into the node accumulators one element colour at a time (no two elements of
+// a colour share a node), so the summation order is fixed and runs are
+// reproducible.
+#pragma once
+
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+namespace nsm {
+
+constexpr int     kTileElems    = 256;  // default elements per tile (one thread each)
+constexpr int     kMaxColors    = 32;
+constexpr uint8_t kColorMask    = 0x1f;  // colour bits of TileSet::color
+constexpr uint8_t kRepeatedNode = 0x80;  // element lists one node more than once
+constexpr uint8_t kInteriorNode = 1;
+
+struct TileSet {
+  int                   tile_elems = kTileElems;  // nodes per tile <= 2 * tile_elems
+  int                   num_tiles = 0;
+  std::vector<int64_t>  perm;           // internal element -> original element
+  std::vector<int32_t>  elem_off;       // [T+1] internal element range of each tile
+  std::vector<int32_t>  node_off;       // [T+1] range into nodes
+  std::vector<int32_t>  nodes;          // global node id per tile-node slot (ascending per tile)
+  std::vector<uint8_t>  node_info;      // kInteriorNode per tile-node slot
+  std::vector<uint16_t> lconn;          // [E][8] local node slot of each element node (internal order)
+  std::vector<uint8_t>  color;          // [E] colour | kRepeatedNode (internal order)
+  std::vector<uint8_t>  num_colors;     // [T]
+  std::vector<int32_t>  bnd_nodes;      // boundary nodes, ascending
+  std::vector<int32_t>  bnd_ptr;        // [B+1]
+  std::vector<int32_t>  bnd_ent;        // tile-node slot of each partial, ascending tile
+  int64_t               num_interior = 0;
+};
+
+// Builds the tiles of blocks given by element offsets blk_off (conn in
+// original element order, [E][8]).  Returns false with a message when a tile
+// cannot be coloured with kMaxColors colours.
+bool build_tiles(int64_t num_nodes, const double* X, const int* conn, const std::vector<int64_t>& blk_off,
+                 int tile_elems, TileSet& ts, std::string* err);
+
+}  // namespace nsm

@@ -210,9 +210,22 @@ class ModelData:
         return out
 
     def DevicePointer(self, label: str):
+        """(device address, bytes per word); integration-point fields are
+        [comp][ipt][E] in the order given by ElementOrder()."""
         p, eb = C.c_void_p(), C.c_int()
         check(lib().nsm_model_field_device_pointer(self._h, FIELDS[label], C.byref(p), C.byref(eb)))
         return p.value, eb.value
+
+    def ElementOrder(self) -> np.ndarray:
+        """Mesh element stored at each device element position."""
+        out = np.zeros(self.num_elem, dtype=np.int64)
+        check(lib().nsm_model_get_element_order(self._h, out.ctypes.data))
+        return out
+
+    def AssemblyInfo(self):
+        t, a, b, c = C.c_int(), C.c_int64(), C.c_int64(), C.c_int64()
+        check(lib().nsm_model_assembly_info(self._h, C.byref(t), C.byref(a), C.byref(b), C.byref(c)))
+        return dict(tiled=bool(t.value), num_tiles=a.value, interior_nodes=b.value, boundary_nodes=c.value)
 
     # -- reaction-force history
     def SetReactionNodes(self, nodes, coordinate=2):

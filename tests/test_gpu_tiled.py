@@ -1,0 +1,161 @@
+"""Tile-local assembly of the mixed-precision path against the FP64 oracle
+and against the element-slot + node-gather mixed path.
+
+Stated bounds (norm-wise relative to the FP64 oracle): one evaluation of the
+internal force <= 1e-5, F <= 1e-6, sigma <= 1e-5; explicit runs: displacement
+<= 1e-6, velocity, internal force and reaction history <= 1e-5.  Between the
+tiled and the slot mixed paths only the FP64 summation order of the same FP32
+element forces differs: internal force within 1e-12 for one evaluation.
+"""
+import numpy as np
+import pytest
+
+import nimblesm_amd as N
+from oracle import oracle as O
+from tests.helpers import oracle_model_from_problem, rel_err
+
+pytestmark = pytest.mark.gpu
+
+
+def mesh_model(X, conn, blk_off, mats, precision, bcs, reaction=None):
+    blocks = [N.Block(b + 1, conn[blk_off[b]:blk_off[b + 1]], mats[b]) for b in range(len(mats))]
+    m = N.ModelData(X, blocks, precision)
+    for kind, nodes, c, v in bcs:
+        m.AddBoundaryCondition(kind, nodes, c, v)
+    if reaction is not None:
+        m.SetReactionNodes(reaction, 2)
+    return m
+
+
+def oracle_model(X, conn, blk_off, mats, bcs):
+    kinds = {"initial_velocity": 0, "prescribed_velocity": 1, "prescribed_displacement": 2}
+    bn, bc, bk, bv = [], [], [], []
+    for kind, nodes, c, v in bcs:
+        for n in nodes:
+            bn.append(n), bc.append(c), bk.append(kinds[kind]), bv.append(v)
+    return O.Model(X, conn, np.asarray(blk_off), np.array([m.density for m in mats]),
+                   np.array([m.bulk_modulus for m in mats]), np.array([m.shear_modulus for m in mats]),
+                   np.array(bn, np.int32), np.array(bc, np.int32), np.array(bk, np.int32), np.array(bv))
+
+
+def cube_bcs(X, v_top=1.0):
+    z = X[:, 2]
+    bottom = np.nonzero(z == z.min())[0].astype(np.int32)
+    top = np.nonzero(z == z.max())[0].astype(np.int32)
+    bcs = [("prescribed_velocity", bottom, c, 0.0) for c in range(3)]
+    bcs.append(("prescribed_velocity", top, 2, v_top))
+    return bcs, top
+
+
+def run_both(X, conn, blk_off, mats, steps, precision="mixed"):
+    bcs, top = cube_bcs(X)
+    m = mesh_model(X, conn, blk_off, mats, precision, bcs, top)
+    om = oracle_model(X, conn, blk_off, mats, bcs)
+    m.ComputeLumpedMass()
+    dt = 0.5 * m.GetCriticalTimeStep()
+    integ = N.ExplicitTimeIntegrator(m, steps * dt, steps, 0).Initialize()
+    integ.Advance(steps)
+    run = O.ExplicitRun(om, steps * dt, steps, 0, threads=8, reaction_nodes=top, reaction_coord=2)
+    run.advance(steps)
+    return m, run
+
+
+def test_assembly_info_and_element_order(hip_device):
+    prob = N.CubeProblem(n=16, materials=[N.problems.STEEL, N.problems.ALUMINIUM], num_steps=1)
+    mixed = prob.make_model("mixed")
+    info = mixed.AssemblyInfo()
+    assert info["tiled"] and info["num_tiles"] == 16
+    assert info["interior_nodes"] + info["boundary_nodes"] == prob.num_nodes
+    order = mixed.ElementOrder()
+    assert np.array_equal(np.sort(order), np.arange(prob.num_elem))
+    half = prob.num_elem // 2  # tiles keep elements inside their block
+    assert order[:half].max() < half and order[half:].min() >= half
+    fp64 = prob.make_model("fp64")
+    assert not fp64.AssemblyInfo()["tiled"]
+    assert np.array_equal(fp64.ElementOrder(), np.arange(prob.num_elem))
+
+
+def test_tiled_internal_force_against_oracle_and_slots(hip_device, monkeypatch):
+    prob = N.CubeProblem(n=12, materials=[N.problems.STEEL, N.problems.ALUMINIUM], num_steps=1, seed=11)
+    rng = np.random.default_rng(4)
+    u = rng.normal(0, 1e-3, (prob.num_nodes, 3))
+    out = {}
+    for mode in ("tiles", "slots"):
+        monkeypatch.setenv("NSM_MIXED_ASSEMBLY", mode)
+        m = prob.make_model("mixed")
+        assert m.AssemblyInfo()["tiled"] == (mode == "tiles")
+        m.SetVectorNodeData("displacement", u)
+        m.ComputeInternalForce()
+        out[mode] = (m.GetVectorNodeData("internal_force"), m.GetIntegrationPointData("deformation_gradient"),
+                     m.GetIntegrationPointData("stress"))
+        m.close()
+    f, Fip, sip = oracle_model_from_problem(prob).internal_force(u, want_ip=True)
+    ft, Ft, st = out["tiles"]
+    print(f"tiled vs oracle: f {rel_err(ft, f):.2e} F {rel_err(Ft, Fip):.2e} sigma {rel_err(st, sip):.2e}")
+    assert rel_err(ft, f) <= 1e-5 and rel_err(Ft, Fip) <= 1e-6 and rel_err(st, sip) <= 1e-5
+    fs, Fs, ss = out["slots"]
+    assert rel_err(ft, fs) <= 1e-12
+    assert rel_err(Ft, Fs) <= 1e-6 and rel_err(st, ss) <= 1e-6
+
+
+def test_tiled_run_on_renumbered_shuffled_mesh(hip_device):
+    """Random node numbering and element order (tiles are then built from
+    scattered ids); 60 steps against the oracle."""
+    n = 10
+    X, conn = N.jittered_cube(n, 21, 0.1)
+    rng = np.random.default_rng(8)
+    new_id = rng.permutation(len(X))
+    Xn = np.zeros_like(X)
+    Xn[new_id] = X
+    conn = new_id[conn].astype(np.int32)
+    half = n ** 3 // 2
+    conn = np.concatenate([conn[:half][rng.permutation(half)], conn[half:][rng.permutation(n ** 3 - half)]])
+    mats = [N.NeohookeanMaterial.from_young(7800, 200e9, 0.3), N.NeohookeanMaterial.from_young(2700, 70e9, 0.33)]
+    steps = 60
+    m, run = run_both(Xn, conn, [0, half, n ** 3], mats, steps)
+    assert m.AssemblyInfo()["tiled"]
+    eu = rel_err(m.GetVectorNodeData("displacement"), run.u)
+    ev = rel_err(m.GetVectorNodeData("velocity"), run.v)
+    ef = rel_err(m.GetVectorNodeData("internal_force"), run.f)
+    eR = rel_err(m.GetReactionHistory(0, steps), run.reaction)
+    print(f"shuffled mesh, tiled mixed vs oracle: u {eu:.2e} v {ev:.2e} f {ef:.2e} R {eR:.2e}")
+    assert eu <= 1e-6 and ev <= 1e-5 and ef <= 1e-5 and eR <= 1e-5
+
+
+def test_tiled_run_with_ten_blocks(hip_device):
+    """More blocks than one element launch holds (8): two launch groups."""
+    n = 10
+    X, conn = N.jittered_cube(n, 5, 0.1)
+    off = [k * n * n for k in range(n + 1)]  # one z-layer per block
+    base = [(7800, 200e9, 0.3), (2700, 70e9, 0.33), (4500, 110e9, 0.34), (1200, 3e9, 0.45)]
+    mats = [N.NeohookeanMaterial.from_young(*base[b % 4]) for b in range(n)]
+    steps = 40
+    m, run = run_both(X, conn, off, mats, steps)
+    assert m.AssemblyInfo()["tiled"]
+    eu = rel_err(m.GetVectorNodeData("displacement"), run.u)
+    ef = rel_err(m.GetVectorNodeData("internal_force"), run.f)
+    eR = rel_err(m.GetReactionHistory(0, steps), run.reaction)
+    print(f"ten blocks, tiled mixed vs oracle: u {eu:.2e} f {ef:.2e} R {eR:.2e}")
+    assert eu <= 1e-6 and ef <= 1e-5 and eR <= 1e-5
+    # the last step evaluates F and sigma at the final displacement; returned in mesh order
+    _, Fo, so = run.model.internal_force(run.u, want_ip=True)
+    eF = rel_err(m.GetIntegrationPointData("deformation_gradient"), Fo)
+    es = rel_err(m.GetIntegrationPointData("stress"), so)
+    print(f"ten blocks: F {eF:.2e} sigma {es:.2e}")
+    assert eF <= 1e-6 and es <= 1e-5
+
+
+def test_tiled_matches_slots_over_a_run(hip_device, monkeypatch):
+    prob = N.CubeProblem(n=16, materials=[N.problems.STEEL, N.problems.ALUMINIUM], num_steps=200)
+    res = {}
+    for mode in ("tiles", "slots"):
+        monkeypatch.setenv("NSM_MIXED_ASSEMBLY", mode)
+        m = prob.make_model("mixed")
+        integ = prob.make_integrator(m)
+        integ.Integrate()
+        res[mode] = (m.GetVectorNodeData("displacement"), m.GetReactionHistory(0, prob.num_steps))
+        m.close()
+    eu = rel_err(res["tiles"][0], res["slots"][0])
+    eR = rel_err(res["tiles"][1], res["slots"][1])
+    print(f"tiled vs slots after 200 steps: u {eu:.2e} R {eR:.2e}")
+    assert eu <= 1e-7 and eR <= 1e-6
