@@ -170,10 +170,18 @@ int nsm_model_explicit_steps(nsm_model* model, int64_t step_begin, int64_t step_
 int nsm_model_get_field(nsm_model* model, int field, double* out);
 int nsm_model_set_field(nsm_model* model, int field, const double* in);
 /* Device address of a field (node fields double; ipt fields double in FP64
- * mode and float in mixed mode) and its element size in bytes.  Ipt fields
+ * mode and float in mixed mode) and its element size in bytes.  Node fields
+ * are [N][3] / [N] in the node order of nsm_model_get_node_order; ipt fields
  * are element-fastest [comp][ipt][E] in the element order of
  * nsm_model_get_element_order. */
 int nsm_model_field_device_pointer(nsm_model* model, int field, void** ptr, int* elem_bytes);
+/* Node order of the device node arrays: out[i] (i < N) is the mesh node
+ * stored at device position i.  FP64 models keep the mesh numbering; tiled
+ * mixed models store each element tile's interior nodes contiguously, then
+ * the nodes shared between tiles.  Host copies (get/set_field) and node lists
+ * (boundary conditions, reaction nodes) always use mesh numbering.  No
+ * reference counterpart: the reference's node views keep the mesh order. */
+int nsm_model_get_node_order(nsm_model* model, int64_t* out);
 /* Element order of the device ipt arrays: out[i] (i < E) is the element, as
  * numbered by the conn given to nsm_model_create, stored at device position
  * i.  FP64 models keep that order; tiled mixed models store each block's

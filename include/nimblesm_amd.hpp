@@ -147,6 +147,22 @@ class ModelData {
     return out;
   }
   void SetVectorNodeData(nsm_field f, const std::vector<double>& v) { check(nsm_model_set_field(h_, f, v.data())); }
+  // F ([E][8][9]) or sigma ([E][8][6]) at the 8 Gauss points, mesh element order
+  std::vector<double> GetIntegrationPointData(nsm_field f)
+  {
+    const int64_t width = f == NSM_FIELD_DEFORMATION_GRADIENT ? 9 : 6;
+    std::vector<double> out((size_t)(num_elem_ * 8 * width));
+    check(nsm_model_get_field(h_, f, out.data()));
+    return out;
+  }
+  // mesh element stored at each device element position (layout of the
+  // device ipt arrays returned by nsm_model_field_device_pointer)
+  std::vector<int64_t> ElementOrder()
+  {
+    std::vector<int64_t> out((size_t)num_elem_);
+    check(nsm_model_get_element_order(h_, out.data()));
+    return out;
+  }
   void SetReactionNodes(const std::vector<int>& nodes, int coordinate)
   {
     check(nsm_model_set_reaction_nodes(h_, (int64_t)nodes.size(), nodes.data(), coordinate));

@@ -186,6 +186,16 @@ struct nsm_model {
 
 namespace {
 
+// original element (conn order) stored at device element position e
+inline int64_t element_of(const nsm_model* md, int64_t e)
+{
+  return md->tiled ? md->tiles.perm[(size_t)e] : e;
+}
+
+// device index of mesh node n, and mesh node at device index i
+inline int64_t dev_node(const nsm_model* md, int64_t n) { return md->tiled ? md->tiles.node_new[(size_t)n] : n; }
+inline int64_t mesh_node(const nsm_model* md, int64_t i) { return md->tiled ? md->tiles.node_old[(size_t)i] : i; }
+
 int ensure_device(nsm_model* md)
 {
   NSM_HIP(hipSetDevice(md->device));
@@ -257,7 +267,7 @@ int rebuild_bcs(nsm_model* md)
       idx = (int)table.size();
       table.push_back(e);
     }
-    code[(size_t)touched[t]] = (uint8_t)(idx + 1);
+    code[(size_t)dev_node(md, touched[t])] = (uint8_t)(idx + 1);
   }
   if (md->bc_table) {
     NSM_HIP(hipFree(md->bc_table));
@@ -270,8 +280,8 @@ int rebuild_bcs(nsm_model* md)
     NSM_HIP(hipMemcpy(md->bc_table, table.data(), table.size() * sizeof(nsm::BcEntry), hipMemcpyHostToDevice));
   std::vector<int> tiles;
   const bool       have_rx = md->reaction_mask.size() == code.size();
-  for (size_t i = 0; i < code.size(); i++)
-    if (have_rx && md->reaction_mask[i]) {
+  for (size_t i = 0; i < code.size(); i++)  // device node order
+    if (have_rx && md->reaction_mask[(size_t)mesh_node(md, (int64_t)i)]) {
       code[i] |= nsm::kReactionFlag;
       const int t = (int)(i / 256);  // node tile of the node kernel
       if (!md->tiled && (tiles.empty() || tiles.back() != t)) tiles.push_back(t);
@@ -282,12 +292,13 @@ int rebuild_bcs(nsm_model* md)
     const nsm::TileSet& ts = md->tiles;
     for (int t = 0; t < ts.num_tiles; t++)
       for (int s = ts.node_off[(size_t)t]; s < ts.node_off[(size_t)t + 1]; s++)
-        if ((ts.node_info[(size_t)s] & nsm::kInteriorNode) && md->reaction_mask[(size_t)ts.nodes[(size_t)s]]) {
+        if ((ts.node_info[(size_t)s] & nsm::kInteriorNode) &&
+            md->reaction_mask[(size_t)mesh_node(md, ts.nodes[(size_t)s])]) {
           tiles.push_back(t);
           break;
         }
     for (size_t b = 0; b < ts.bnd_nodes.size(); b++)
-      if (md->reaction_mask[(size_t)ts.bnd_nodes[b]]) {
+      if (md->reaction_mask[(size_t)mesh_node(md, ts.bnd_nodes[b])]) {
         const int g = ts.num_tiles + (int)(b / nsm::kBoundaryNodesPerGroup);
         if (tiles.empty() || tiles.back() != g) tiles.push_back(g);
       }
@@ -460,6 +471,7 @@ int setup_tiles(nsm_model* md)
         ts.node_off[(size_t)t + 1] - ts.node_off[(size_t)t] > 2 * ts.tile_elems)
       return fail(NSM_ERR_ARG, "element tile %d exceeds the kernel's tile size", t);
   ta.tile_elems          = ts.tile_elems;
+  ta.bnd_base            = (int)ts.num_interior;
   ta.num_tiles           = ts.num_tiles;
   int cus                = 0;
   NSM_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, md->device));
@@ -467,13 +479,13 @@ int setup_tiles(nsm_model* md)
   ta.num_bnd             = (int)ts.bnd_nodes.size();
   int st                 = 0;
   if ((st = upload(md, ts.elem_off, &ta.elem_off)) || (st = upload(md, ts.node_off, &ta.node_off)) ||
-      (st = upload(md, ts.nodes, &ta.nodes)) || (st = upload(md, ts.node_info, &ta.node_info)) ||
+      (st = upload(md, ts.nodes, &ta.nodes)) || (st = upload(md, ts.slot_dst, &ta.slot_dst)) ||
       (st = upload(md, ts.lconn, &ta.lconn)) || (st = upload(md, ts.color, &ta.color)) ||
-      (st = upload(md, std::vector<int32_t>(ts.num_colors.begin(), ts.num_colors.end()), &ta.num_colors)) || (st = upload(md, ts.bnd_nodes, &ta.bnd_nodes)) ||
-      (st = upload(md, ts.bnd_ptr, &ta.bnd_ptr)) || (st = upload(md, ts.bnd_ent, &ta.bnd_ent)))
+      (st = upload(md, std::vector<int32_t>(ts.num_colors.begin(), ts.num_colors.end()), &ta.num_colors)) ||
+      (st = upload(md, ts.bnd_ptr, &ta.bnd_ptr)))
     return st;
   double* partials = nullptr;
-  if ((st = dalloc(&partials, std::max<size_t>(3 * ts.nodes.size(), 1), &md->device_bytes))) return st;
+  if ((st = dalloc(&partials, std::max<size_t>(3 * ts.bnd_ent.size(), 1), &md->device_bytes))) return st;
   md->tile_bufs.push_back((void*)partials);
   ta.partials = partials;
   md->blk_tile_off.assign((size_t)md->num_blocks + 1, 0);
@@ -484,11 +496,6 @@ int setup_tiles(nsm_model* md)
   return NSM_OK;
 }
 
-// original element (conn order) stored at device element position e
-inline int64_t element_of(const nsm_model* md, int64_t e)
-{
-  return md->tiled ? md->tiles.perm[(size_t)e] : e;
-}
 
 nsm::StepTimes step_times(double* time_current, double dt_user)
 {
@@ -625,16 +632,6 @@ int nsm_model_create(int64_t num_nodes, const double* reference_coordinates, int
     if (conn[k] < 0 || conn[k] >= num_nodes)
       return fail(NSM_ERR_ARG, "connectivity entry %lld = %d out of range", (long long)k, conn[k]);
 
-  // node -> (element e, local node j) incidence in ascending element order,
-  // stored as the slot offset j*E + e of the element-fastest force slots
-  std::vector<uint32_t> ptr((size_t)num_nodes + 1, 0), ent((size_t)(8 * E));
-  for (int64_t k = 0; k < 8 * E; k++) ptr[(size_t)conn[k] + 1]++;
-  for (int64_t i = 0; i < num_nodes; i++) ptr[(size_t)i + 1] += ptr[(size_t)i];
-  {
-    std::vector<uint32_t> fill(ptr.begin(), ptr.end() - 1);
-    for (int64_t k = 0; k < 8 * E; k++) ent[fill[(size_t)conn[k]]++] = (uint32_t)((k & 7) * E + (k >> 3));
-  }
-
   nsm_model* md  = new nsm_model();
   md->precision  = precision;
   md->N          = num_nodes;
@@ -669,6 +666,34 @@ int nsm_model_create(int64_t num_nodes, const double* reference_coordinates, int
     }
   }
 
+  // Device node numbering: the mesh's, or the tiled model's (tile-interior
+  // nodes contiguous per tile, then the nodes shared between tiles).
+  std::vector<double> X_dev;
+  std::vector<int>    conn_dev;
+  const double*       Xd = reference_coordinates;
+  const int*          cd = conn;
+  if (md->tiled) {
+    const auto& old_of = md->tiles.node_old;
+    const auto& new_of = md->tiles.node_new;
+    X_dev.resize(3 * (size_t)num_nodes);
+    for (int64_t i = 0; i < num_nodes; i++)
+      for (int c = 0; c < 3; c++) X_dev[(size_t)(3 * i + c)] = reference_coordinates[3 * (int64_t)old_of[(size_t)i] + c];
+    conn_dev.resize((size_t)(8 * E));
+    for (int64_t k = 0; k < 8 * E; k++) conn_dev[(size_t)k] = new_of[(size_t)conn[k]];
+    Xd = X_dev.data();
+    cd = conn_dev.data();
+  }
+
+  // node -> (element e, local node j) incidence in ascending element order,
+  // stored as the slot offset j*E + e of the element-fastest force slots
+  std::vector<uint32_t> ptr((size_t)num_nodes + 1, 0), ent((size_t)(8 * E));
+  for (int64_t k = 0; k < 8 * E; k++) ptr[(size_t)cd[k] + 1]++;
+  for (int64_t i = 0; i < num_nodes; i++) ptr[(size_t)i + 1] += ptr[(size_t)i];
+  {
+    std::vector<uint32_t> fill(ptr.begin(), ptr.end() - 1);
+    for (int64_t k = 0; k < 8 * E; k++) ent[fill[(size_t)cd[k]]++] = (uint32_t)((k & 7) * E + (k >> 3));
+  }
+
   const size_t N3 = 3 * (size_t)num_nodes;
   const size_t ipw = precision == NSM_FP64 ? sizeof(double) : sizeof(float);
   size_t&      tot = md->device_bytes;
@@ -695,9 +720,9 @@ int nsm_model_create(int64_t num_nodes, const double* reference_coordinates, int
     hipError_t _e = (call);                                                                        \
     if (_e != hipSuccess) return cleanup(fail(NSM_ERR_HIP, "%s: %s", #call, hipGetErrorString(_e))); \
   } while (0)
-  NSM_HIP_C(hipMemcpyAsync(md->X, reference_coordinates, N3 * sizeof(double), hipMemcpyHostToDevice, s));
+  NSM_HIP_C(hipMemcpyAsync(md->X, Xd, N3 * sizeof(double), hipMemcpyHostToDevice, s));
   if (E > 0) {
-    NSM_HIP_C(hipMemcpyAsync(md->conn, conn, (size_t)(8 * E) * sizeof(int), hipMemcpyHostToDevice, s));
+    NSM_HIP_C(hipMemcpyAsync(md->conn, cd, (size_t)(8 * E) * sizeof(int), hipMemcpyHostToDevice, s));
     NSM_HIP_C(hipMemcpyAsync(md->node_ent, ent.data(), ent.size() * sizeof(uint32_t), hipMemcpyHostToDevice, s));
   }
   NSM_HIP_C(hipMemcpyAsync(md->node_ptr, ptr.data(), ptr.size() * sizeof(uint32_t), hipMemcpyHostToDevice, s));
@@ -793,7 +818,7 @@ int nsm_model_apply_initial_conditions(nsm_model* md)
   NSM_HIP(hipStreamSynchronize(md->stream));
   for (const auto& b : md->bcs)
     if (b.kind == NSM_BC_INITIAL_VELOCITY)
-      for (int n : b.nodes) v[(size_t)(3 * (int64_t)n + b.coord)] = b.value;
+      for (int n : b.nodes) v[(size_t)(3 * dev_node(md, n) + b.coord)] = b.value;
   NSM_HIP(hipMemcpyAsync(md->v, v.data(), v.size() * sizeof(double), hipMemcpyHostToDevice, md->stream));
   NSM_HIP(hipStreamSynchronize(md->stream));
   return NSM_OK;
@@ -1003,8 +1028,18 @@ int nsm_model_get_field(nsm_model* md, int field, double* out)
     }
     default: return fail(NSM_ERR_ARG, "unknown field %d", field);
   }
-  NSM_HIP(hipMemcpyAsync(out, src, cnt * sizeof(double), hipMemcpyDeviceToHost, md->stream));
+  if (!md->tiled) {
+    NSM_HIP(hipMemcpyAsync(out, src, cnt * sizeof(double), hipMemcpyDeviceToHost, md->stream));
+    NSM_HIP(hipStreamSynchronize(md->stream));
+    return NSM_OK;
+  }
+  // device node order -> mesh node order
+  std::vector<double> tmp(cnt);
+  NSM_HIP(hipMemcpyAsync(tmp.data(), src, cnt * sizeof(double), hipMemcpyDeviceToHost, md->stream));
   NSM_HIP(hipStreamSynchronize(md->stream));
+  const size_t w = cnt / (size_t)md->N;
+  for (int64_t n = 0; n < md->N; n++)
+    for (size_t c = 0; c < w; c++) out[(size_t)n * w + c] = tmp[(size_t)dev_node(md, n) * w + c];
   return NSM_OK;
 }
 
@@ -1028,7 +1063,17 @@ int nsm_model_set_field(nsm_model* md, int field, const double* in)
       break;
     default: return fail(NSM_ERR_ARG, "field %d cannot be set", field);
   }
-  NSM_HIP(hipMemcpyAsync(dst, in, cnt * sizeof(double), hipMemcpyHostToDevice, md->stream));
+  if (!md->tiled) {
+    NSM_HIP(hipMemcpyAsync(dst, in, cnt * sizeof(double), hipMemcpyHostToDevice, md->stream));
+    NSM_HIP(hipStreamSynchronize(md->stream));
+    return NSM_OK;
+  }
+  // mesh node order -> device node order
+  std::vector<double> tmp(cnt);
+  const size_t        w = cnt / (size_t)md->N;
+  for (int64_t n = 0; n < md->N; n++)
+    for (size_t c = 0; c < w; c++) tmp[(size_t)dev_node(md, n) * w + c] = in[(size_t)n * w + c];
+  NSM_HIP(hipMemcpyAsync(dst, tmp.data(), cnt * sizeof(double), hipMemcpyHostToDevice, md->stream));
   NSM_HIP(hipStreamSynchronize(md->stream));
   return NSM_OK;
 }
@@ -1091,6 +1136,13 @@ int nsm_model_kernel_time(nsm_model* md, int kernel, double* avg_ms, int64_t* la
   }
   *avg_ms = n ? sum / (double)n : 0.0;
   if (launches) *launches = n;
+  return NSM_OK;
+}
+
+int nsm_model_get_node_order(nsm_model* md, int64_t* out)
+{
+  if (!md || !out) return fail(NSM_ERR_ARG, "null argument");
+  for (int64_t i = 0; i < md->N; i++) out[i] = mesh_node(md, i);
   return NSM_OK;
 }
 

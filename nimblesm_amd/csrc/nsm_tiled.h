@@ -1,7 +1,8 @@
 // nsm_tiled.h -- launchers of the tile-local mixed-precision step (internal).
 //
-// Device arrays of a TileSet (nsm_tiles.h) plus the FP64 partial force of
-// every tile-node slot that is not interior to its tile.
+// Device arrays of a TileSet (nsm_tiles.h) plus the FP64 partial forces of
+// the tile-node slots that are not interior to their tile, stored by node
+// (the partials of one node are contiguous, in ascending tile order).
 #pragma once
 
 #include "nsm_kernels.h"
@@ -16,14 +17,13 @@ struct TileArgs {
   const int32_t*  elem_off;    // [T+1]
   const int32_t*  node_off;    // [T+1]
   const int32_t*  nodes;       // [slots]
-  const uint8_t*  node_info;   // [slots]
+  const int32_t*  slot_dst;    // [slots] -1: interior node, else index of its partial
   const uint16_t* lconn;       // [E][8], internal element order
   const uint8_t*  color;       // [E]
   const int32_t*  num_colors;  // [T]
-  const int32_t*  bnd_nodes;   // [B]
+  int             bnd_base;    // device id of boundary node 0 (boundary node b is bnd_base + b)
   const int32_t*  bnd_ptr;     // [B+1]
-  const int32_t*  bnd_ent;     // partial slots, ascending tile per node
-  double*         partials;    // [slots][3]
+  double*         partials;    // [partials][3]; node b's are [bnd_ptr[b], bnd_ptr[b+1])
 };
 
 // boundary nodes per workgroup of the boundary kernel (one per thread)

@@ -54,13 +54,15 @@ __device__ __forceinline__ T tile_table(const T* p, int t)
   return ((const __attribute__((address_space(4))) T*)p)[t];
 }
 
-// Node ids of the (at most two) tile-node slots a thread stages.  A thread
+// Node ids of the (at most two) tile-node slots a thread stages, with the
+// slot's partial destination (kInterior for a tile-interior node).  A thread
 // past the tile's last slot gets that last slot's node (so every load stays
-// unconditional) with kAbsent set.
-constexpr int kAbsent = 0x100;
+// unconditional) with destination kAbsent.
+constexpr int kInterior = -1;
+constexpr int kAbsent   = -2;
 struct SlotIds {
   int gid[2];
-  int info[2];  // node_info | kAbsent
+  int dst[2];
 };
 
 template <int kThreads>
@@ -74,7 +76,7 @@ __device__ __forceinline__ SlotIds load_ids(const TileArgs& ta, int t)
     const int ln = (int)threadIdx.x + k * kThreads;
     const int sl = n0 + min(ln, nn - 1);
     r.gid[k]     = ta.nodes[sl];
-    r.info[k]    = ta.node_info[sl] | (ln < nn ? 0 : kAbsent);
+    r.dst[k]     = ln < nn ? ta.slot_dst[sl] : kAbsent;
   }
   return r;
 }
@@ -117,7 +119,7 @@ __device__ __forceinline__ void stage(const SlotIds& ids, const double (&xr)[2][
 #pragma unroll
   for (int k = 0; k < 2; k++) {
     const int ln = (int)threadIdx.x + k * kThreads;
-    if (!(ids.info[k] & kAbsent)) {
+    if (ids.dst[k] != kAbsent) {
 #pragma unroll
       for (int c = 0; c < 3; c++) {
         sX[ln][c]  = xr[k][c];
@@ -125,7 +127,7 @@ __device__ __forceinline__ void stage(const SlotIds& ids, const double (&xr)[2][
         acc[ln][c] = 0.0;
       }
     }
-    if ((ids.info[k] & (kInteriorNode | kAbsent)) == kInteriorNode) {
+    if (ids.dst[k] == kInterior) {
       ns.flg[k] = na.flags[ids.gid[k]];
       if (sa.update_v) {
         load3(na.v, ids.gid[k], ns.v[k]);
@@ -135,11 +137,10 @@ __device__ __forceinline__ void stage(const SlotIds& ids, const double (&xr)[2][
   }
 }
 
-// Persistent workgroups, each walking a contiguous range of tiles (the
-// workgroups of one XCD cover one contiguous range of the Morton-ordered
-// tiles).  kThreads elements and at most 2*kThreads nodes per tile.  While
-// tile t computes, X, u and element data of tile t+1 and node ids of tile t+2
-// are in flight; tile t+1 is staged into LDS at the end of the iteration.
+// Persistent workgroups, each walking every G-th tile.  kThreads elements
+// and at most 2*kThreads nodes per tile.  While tile t computes, X, u and
+// element data of its next tile and node ids of the one after are in flight;
+// the next tile is staged into LDS at the end of the iteration.
 // Inside the loop every vector load and every F / sigma store is executed on
 // every path (indices clamped), so at least the 120 stores of the element
 // arithmetic always separate a load from its first use and the waits the
@@ -159,18 +160,27 @@ __global__ __launch_bounds__(kThreads) void k_element_tiled(const BlockTable<flo
   __shared__ BcEntry sbc[kMaxBcEntries];
   __shared__ int     bad;
 
-  const int w  = (int)xcd_tile(blockIdx.x, gridDim.x);
-  const int tb = uniform(t_begin + (int)((int64_t)t_count * w / gridDim.x));
-  const int te = uniform(t_begin + (int)((int64_t)t_count * (w + 1) / gridDim.x));
-  if (tb >= te) return;  // uniform across the workgroup
-  const int tid = (int)threadIdx.x;
+  // Workgroup w (numbered so that each XCD holds a contiguous run of w)
+  // processes tiles t_begin + w, + G, + 2G, ... (G workgroups): at any time
+  // the grid works on one window of G consecutive tiles, so the F and sigma
+  // rows are written in one narrow address range and each XCD's L2 sees
+  // neighbouring tiles.
+  const int G    = (int)gridDim.x;
+  const int w    = (int)xcd_tile(blockIdx.x, gridDim.x);
+  const int tend = t_begin + t_count;
+  const int tb   = uniform(t_begin + w);
+  if (tb >= tend) return;  // uniform across the workgroup
+  // last tile of this workgroup, and the tile after t (t itself if none)
+  const int  te   = uniform(tb + ((tend - 1 - tb) / G) * G + 1);
+  const auto next = [&](int t) { return t + G < te ? t + G : t; };
+  const int  tid  = (int)threadIdx.x;
 
   // ---- prologue: stage tile tb, load tile tb+1's node ids
   if (tid == 0) bad = 0;
   if (sa.update_v)
     for (int b = tid; b < na.num_bc; b += kThreads) sbc[b] = na.bc_table[b];
   SlotIds ids      = load_ids<kThreads>(ta, tb);
-  SlotIds ids_next = load_ids<kThreads>(ta, min(tb + 1, te - 1));
+  SlotIds ids_next = load_ids<kThreads>(ta, next(tb));
   double  xr[2][3], ur[2][3];
 #pragma unroll
   for (int k = 0; k < 2; k++) {
@@ -184,22 +194,21 @@ __global__ __launch_bounds__(kThreads) void k_element_tiled(const BlockTable<flo
   // maximum, gfx9 encoding): the loop's waits then depend on its body only
   __builtin_amdgcn_s_waitcnt(0x0F70);
 
-  for (int tl = tb; tl < te; tl++) {
+  for (int tl = tb; tl < te; tl += G) {
     const int t    = uniform(tl);
     const int e0   = tile_table(ta.elem_off, t);
-    const int n0   = tile_table(ta.node_off, t);
     const int ncol = tile_table(ta.num_colors, t);
     __syncthreads();  // tile t staged
 
-    // ---- prefetch (clamped to the last tile): tile t+1's nodes and
-    // elements, tile t+2's node ids
+    // ---- prefetch (clamped to this workgroup's last tile): the next tile's
+    // nodes and elements, the node ids of the one after
 #pragma unroll
     for (int k = 0; k < 2; k++) {
       load3(X, ids_next.gid[k], xr[k]);
       load3(na.u, ids_next.gid[k], ur[k]);
     }
-    const ElemData el_next   = load_elem<kThreads>(ta, min(t + 1, te - 1));
-    const SlotIds  ids_after = load_ids<kThreads>(ta, min(t + 2, te - 1));
+    const ElemData el_next   = load_elem<kThreads>(ta, next(t));
+    const SlotIds  ids_after = load_ids<kThreads>(ta, next(next(t)));
 
     // ---- element arithmetic (every thread; see load_elem)
     const int64_t e = e0 + min(tid, tile_table(ta.elem_off, t + 1) - e0 - 1);
@@ -263,14 +272,14 @@ __global__ __launch_bounds__(kThreads) void k_element_tiled(const BlockTable<flo
 #pragma unroll
     for (int k = 0; k < 2; k++) {
       const int ln = tid + k * kThreads;
-      if (ids.info[k] & kAbsent) continue;
+      if (ids.dst[k] == kAbsent) continue;
       double fn[3] = {acc[ln][0], acc[ln][1], acc[ln][2]};
-      if (ids.info[k] & kInteriorNode) {
+      if (ids.dst[k] == kInterior) {
         if (rx.coord >= 0 && (ns.flg[k] & kReactionFlag)) rxs += fn[rx.coord];
         double un[3] = {su[ln][0], su[ln][1], su[ln][2]};
         node_update(na, sbc, ids.gid[k], ns.flg[k], fn, un, ns.v[k], ns.m[k], sa);
       } else {
-        store3(ta.partials, (int64_t)(n0 + ln), fn);
+        store3(ta.partials, (int64_t)ids.dst[k], fn);
       }
     }
     if (rx.coord >= 0) {  // uniform across the grid
@@ -279,11 +288,11 @@ __global__ __launch_bounds__(kThreads) void k_element_tiled(const BlockTable<flo
     }
     __syncthreads();  // tile t's LDS reads done
 
-    // ---- stage tile t+1
+    // ---- stage the next tile
     ids      = ids_next;
     ids_next = ids_after;
     cur      = el_next;
-    if (t + 1 < te) stage<kThreads>(ids, xr, ur, na, sa, sX, su, acc, ns);
+    if (t + G < te) stage<kThreads>(ids, xr, ur, na, sa, sX, su, acc, ns);
   }
   if (tid == 0 && bad) atomicOr(status, 1);
 }
@@ -298,14 +307,13 @@ __global__ __launch_bounds__(kBndThreads) void k_boundary_update(TileArgs ta, No
   const int         bi  = (int)g * kBndThreads + threadIdx.x;
   double            rxs = 0.0;
   if (bi < ta.num_bnd) {
-    const int64_t i  = ta.bnd_nodes[bi];
+    const int64_t i  = (int64_t)ta.bnd_base + bi;
     const int     p0 = ta.bnd_ptr[bi], p1 = ta.bnd_ptr[bi + 1];
     double        f[3] = {0.0, 0.0, 0.0};
-    for (int p = p0; p < p1; p++) {
-      const int64_t s = ta.bnd_ent[p];
-      f[0] += ta.partials[3 * s];
-      f[1] += ta.partials[3 * s + 1];
-      f[2] += ta.partials[3 * s + 2];
+    for (int64_t p = p0; p < p1; p++) {
+      f[0] += ta.partials[3 * p];
+      f[1] += ta.partials[3 * p + 1];
+      f[2] += ta.partials[3 * p + 2];
     }
     const int flags = na.flags[i];
     if (rx.coord >= 0 && (flags & kReactionFlag)) rxs = f[rx.coord];

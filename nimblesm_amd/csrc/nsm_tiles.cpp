@@ -3,6 +3,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstddef>
 #include <string>
 
 namespace nsm {
@@ -21,6 +22,8 @@ uint64_t spread3(uint64_t v)
 }
 
 }  // namespace
+
+void renumber_nodes(int64_t num_nodes, TileSet& ts);
 
 bool build_tiles(int64_t num_nodes, const double* X, const int* conn, const std::vector<int64_t>& blk_off,
                  int tile_elems, TileSet& ts, std::string* err)
@@ -134,6 +137,27 @@ bool build_tiles(int64_t num_nodes, const double* X, const int* conn, const std:
         ts.color[(size_t)p] = (uint8_t)(c | (repeated ? kRepeatedNode : 0));
         ncol                = std::max(ncol, c + 1);
       }
+      // Elements of the tile in colour order (stable): a 64-lane wave then
+      // holds few colours and skips the assembly passes of the others.
+      {
+        const size_t           n0 = (size_t)tstart, n = (size_t)(pos - tstart);
+        std::vector<size_t>    idx(n);
+        for (size_t i = 0; i < n; i++) idx[i] = i;
+        std::stable_sort(idx.begin(), idx.end(), [&](size_t a, size_t b) {
+          return (ts.color[n0 + a] & kColorMask) < (ts.color[n0 + b] & kColorMask);
+        });
+        std::vector<int64_t>  perm2(n);
+        std::vector<uint8_t>  color2(n);
+        std::vector<uint16_t> lconn2(8 * n);
+        for (size_t i = 0; i < n; i++) {
+          perm2[i]  = ts.perm[n0 + idx[i]];
+          color2[i] = ts.color[n0 + idx[i]];
+          for (int j = 0; j < 8; j++) lconn2[8 * i + j] = ts.lconn[8 * (n0 + idx[i]) + j];
+        }
+        std::copy(perm2.begin(), perm2.end(), ts.perm.begin() + (std::ptrdiff_t)n0);
+        std::copy(color2.begin(), color2.end(), ts.color.begin() + (std::ptrdiff_t)n0);
+        std::copy(lconn2.begin(), lconn2.end(), ts.lconn.begin() + (std::ptrdiff_t)(8 * n0));
+      }
       for (size_t i = 0; i < tnodes.size(); i++) {
         ts.nodes.push_back(tnodes[i]);
         in_tile_cnt.push_back(cnt[i]);
@@ -175,7 +199,57 @@ bool build_tiles(int64_t num_nodes, const double* X, const int* conn, const std:
       const int32_t bi         = first[(size_t)ts.nodes[s]];
       ts.bnd_ent[(size_t)fill[(size_t)bi]++] = (int32_t)s;
     }
+  renumber_nodes(num_nodes, ts);
   return true;
+}
+
+// Node numbering of the tiled model: the interior nodes of tile 0, of tile 1,
+// ... (each tile's in slot order), then the boundary nodes in ascending
+// original id.  Within every tile the interior slots move to the front.  All
+// node ids of the TileSet become new ids; node_new / node_old map between
+// the two numberings.
+void renumber_nodes(int64_t num_nodes, TileSet& ts)
+{
+  ts.node_new.assign((size_t)num_nodes, -1);
+  int32_t next = 0;
+  ts.int_off.assign((size_t)ts.num_tiles + 1, 0);
+  for (int t = 0; t < ts.num_tiles; t++) {
+    ts.int_off[(size_t)t] = next;
+    for (int32_t sl = ts.node_off[(size_t)t]; sl < ts.node_off[(size_t)t + 1]; sl++)
+      if (ts.node_info[(size_t)sl] & kInteriorNode) ts.node_new[(size_t)ts.nodes[(size_t)sl]] = next++;
+  }
+  ts.int_off[(size_t)ts.num_tiles] = next;
+  for (size_t b = 0; b < ts.bnd_nodes.size(); b++) ts.node_new[(size_t)ts.bnd_nodes[b]] = next++;
+  ts.node_old.assign((size_t)num_nodes, 0);
+  for (int64_t n = 0; n < num_nodes; n++) ts.node_old[(size_t)ts.node_new[(size_t)n]] = (int32_t)n;
+
+  // interior slots first inside each tile; remap the local connectivity
+  std::vector<int32_t> nodes2(ts.nodes.size());
+  std::vector<uint8_t> info2(ts.nodes.size());
+  std::vector<int32_t> global_slot(ts.nodes.size());  // old global slot -> new global slot
+  for (int t = 0; t < ts.num_tiles; t++) {
+    const int32_t s0 = ts.node_off[(size_t)t], s1 = ts.node_off[(size_t)t + 1];
+    int32_t       d  = s0;
+    for (int pass = 0; pass < 2; pass++)
+      for (int32_t sl = s0; sl < s1; sl++)
+        if (((ts.node_info[(size_t)sl] & kInteriorNode) != 0) == (pass == 0)) {
+          global_slot[(size_t)sl] = d;
+          nodes2[(size_t)d]       = ts.node_new[(size_t)ts.nodes[(size_t)sl]];
+          info2[(size_t)d]        = ts.node_info[(size_t)sl];
+          d++;
+        }
+    for (int32_t e = ts.elem_off[(size_t)t]; e < ts.elem_off[(size_t)t + 1]; e++)
+      for (int j = 0; j < 8; j++) {
+        uint16_t& l = ts.lconn[(size_t)(8 * e + j)];
+        l           = (uint16_t)(global_slot[(size_t)(s0 + l)] - s0);
+      }
+  }
+  ts.nodes.swap(nodes2);
+  ts.node_info.swap(info2);
+  for (auto& s : ts.bnd_ent) s = global_slot[(size_t)s];  // still ascending tile order per node
+  for (auto& n : ts.bnd_nodes) n = ts.node_new[(size_t)n];
+  ts.slot_dst.assign(ts.nodes.size(), -1);
+  for (size_t p = 0; p < ts.bnd_ent.size(); p++) ts.slot_dst[(size_t)ts.bnd_ent[p]] = (int32_t)p;
 }
 
 }  // namespace nsm

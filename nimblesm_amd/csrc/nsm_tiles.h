@@ -39,11 +39,21 @@ struct TileSet {
   std::vector<int32_t>  bnd_nodes;      // boundary nodes, ascending
   std::vector<int32_t>  bnd_ptr;        // [B+1]
   std::vector<int32_t>  bnd_ent;        // tile-node slot of each partial, ascending tile
+  std::vector<int32_t>  slot_dst;       // per tile-node slot: -1 interior, else its partial's index p
+                                        // (bnd_ent[p] == slot; partials of a node are contiguous)
   int64_t               num_interior = 0;
+  // Node numbering of the tiled model (every node id above is a new id):
+  // tile t's interior nodes are [int_off[t], int_off[t+1]); boundary node b
+  // (bnd_nodes[b]) is num_interior + b.  Inside a tile the interior slots
+  // come first.
+  std::vector<int32_t>  node_new;       // [N] original id -> new id
+  std::vector<int32_t>  node_old;       // [N] new id -> original id
+  std::vector<int32_t>  int_off;        // [T+1]
 };
 
 // Builds the tiles of blocks given by element offsets blk_off (conn in
-// original element order, [E][8]).  Returns false with a message when a tile
+// original element and node numbering, [E][8]) and renumbers the nodes (see
+// TileSet::node_new).  Returns false with a message when a tile
 // cannot be coloured with kMaxColors colours.
 bool build_tiles(int64_t num_nodes, const double* X, const int* conn, const std::vector<int64_t>& blk_off,
                  int tile_elems, TileSet& ts, std::string* err);

@@ -210,11 +210,18 @@ class ModelData:
         return out
 
     def DevicePointer(self, label: str):
-        """(device address, bytes per word); integration-point fields are
-        [comp][ipt][E] in the order given by ElementOrder()."""
+        """(device address, bytes per word); node fields are in the order
+        given by NodeOrder(), integration-point fields are [comp][ipt][E] in
+        the order given by ElementOrder()."""
         p, eb = C.c_void_p(), C.c_int()
         check(lib().nsm_model_field_device_pointer(self._h, FIELDS[label], C.byref(p), C.byref(eb)))
         return p.value, eb.value
+
+    def NodeOrder(self) -> np.ndarray:
+        """Mesh node stored at each device node position."""
+        out = np.zeros(self.num_nodes, dtype=np.int64)
+        check(lib().nsm_model_get_node_order(self._h, out.ctypes.data))
+        return out
 
     def ElementOrder(self) -> np.ndarray:
         """Mesh element stored at each device element position."""

@@ -97,3 +97,27 @@ def test_single_hip_runtime_in_process():
     hip = set(re.findall(r"(/\S*libamdhip64\S*)", maps))
     hsa = set(re.findall(r"(/\S*libhsa-runtime64\S*)", maps))
     assert len(hip) == 1 and len(hsa) == 1, (hip, hsa)
+
+
+def test_headers_compile_as_c_and_cpp(tmp_path):
+    """include/nimblesm_amd.h is plain C99; include/nimblesm_amd.hpp (the
+    C++ adapters a NimbleSM build would include) compiles as C++17."""
+    import subprocess
+
+    inc = os.path.dirname(HEADER)
+    c_src = tmp_path / "use.c"
+    c_src.write_text('#include "nimblesm_amd.h"\nint f(nsm_model* m) { return nsm_model_synchronize(m); }\n')
+    subprocess.run(["gcc", "-std=c99", "-Wall", "-Wextra", "-Werror", "-fsyntax-only", "-I", inc, str(c_src)],
+                   check=True)
+    cpp_src = tmp_path / "use.cpp"
+    cpp_src.write_text(
+        '#include "nimblesm_amd.hpp"\n'
+        "double f(nimble_amd::ModelData& m) {\n"
+        "  nimble_amd::ExplicitTimeIntegrator integ(m, 0.0, 1.0, 10, 1);\n"
+        "  integ.Initialize(); integ.Advance(2);\n"
+        "  auto F = m.GetIntegrationPointData(NSM_FIELD_DEFORMATION_GRADIENT);\n"
+        "  auto order = m.ElementOrder();\n"
+        "  return F.empty() || order.empty() ? 0.0 : F[0];\n"
+        "}\n")
+    subprocess.run(["g++", "-std=c++17", "-Wall", "-Wextra", "-Werror", "-fsyntax-only", "-I", inc, str(cpp_src)],
+                   check=True)

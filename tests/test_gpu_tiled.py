@@ -70,9 +70,19 @@ def test_assembly_info_and_element_order(hip_device):
     assert np.array_equal(np.sort(order), np.arange(prob.num_elem))
     half = prob.num_elem // 2  # tiles keep elements inside their block
     assert order[:half].max() < half and order[half:].min() >= half
+    # device node order: each tile's interior nodes, then the shared nodes;
+    # host copies stay in mesh order
+    node_order = mixed.NodeOrder()
+    assert np.array_equal(np.sort(node_order), np.arange(prob.num_nodes))
+    rng = np.random.default_rng(0)
+    u = rng.normal(size=(prob.num_nodes, 3))
+    mixed.SetVectorNodeData("displacement", u)
+    assert np.array_equal(mixed.GetVectorNodeData("displacement"), u)
+    assert np.array_equal(mixed.GetVectorNodeData("reference_coordinate"), prob.X)
     fp64 = prob.make_model("fp64")
     assert not fp64.AssemblyInfo()["tiled"]
     assert np.array_equal(fp64.ElementOrder(), np.arange(prob.num_elem))
+    assert np.array_equal(fp64.NodeOrder(), np.arange(prob.num_nodes))
 
 
 def test_tiled_internal_force_against_oracle_and_slots(hip_device, monkeypatch):

@@ -1,9 +1,10 @@
 """Host tile builder of the tile-local mixed-precision assembly
 (nimblesm_amd/csrc/nsm_tiles.cpp), compiled with g++ from its source and
 checked on the CPU: every element lands in exactly one tile of its block,
-the local connectivity maps back to the mesh, colours separate elements that
-share a node, and interior nodes + ordered boundary partials reproduce the
-element-force scatter sum of every node exactly once."""
+the local connectivity maps back to the mesh under the builder's node
+renumbering, colours separate elements that share a node, and interior nodes
++ ordered boundary partials reproduce the element-force scatter sum of every
+node exactly once."""
 import ctypes as C
 import os
 import subprocess
@@ -30,7 +31,7 @@ def tiles_lib(tmp_path_factory):
     lib.tiles_build.restype = C.c_void_p
     lib.tiles_build.argtypes = [C.c_int64, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_int]
     lib.tiles_sizes.argtypes = [C.c_void_p, C.c_void_p]
-    lib.tiles_copy.argtypes = [C.c_void_p] + [C.c_void_p] * 11
+    lib.tiles_copy.argtypes = [C.c_void_p] + [C.c_void_p] * 14
     lib.tiles_free.argtypes = [C.c_void_p]
     return lib
 
@@ -49,21 +50,35 @@ def build(lib, X, conn, blk_off, tile_elems=256):
         perm=np.zeros(E, np.int64), elem_off=np.zeros(T + 1, np.int32), node_off=np.zeros(T + 1, np.int32),
         nodes=np.zeros(S, np.int32), node_info=np.zeros(S, np.uint8), lconn=np.zeros((E, 8), np.uint16),
         color=np.zeros(E, np.uint8), num_colors=np.zeros(T, np.uint8), bnd_nodes=np.zeros(B, np.int32),
-        bnd_ptr=np.zeros(B + 1, np.int32), bnd_ent=np.zeros(BE, np.int32),
+        bnd_ptr=np.zeros(B + 1, np.int32), bnd_ent=np.zeros(BE, np.int32), slot_dst=np.zeros(S, np.int32),
+        node_new=np.zeros(len(X), np.int32), int_off=np.zeros(T + 1, np.int32),
     )
     lib.tiles_copy(h, *(t[k].ctypes.data for k in ("perm", "elem_off", "node_off", "nodes", "node_info", "lconn",
-                                                   "color", "num_colors", "bnd_nodes", "bnd_ptr", "bnd_ent")))
+                                                   "color", "num_colors", "bnd_nodes", "bnd_ptr", "bnd_ent",
+                                                   "slot_dst", "node_new", "int_off")))
     lib.tiles_free(h)
     t["num_interior"] = int(sz[4])
     t["tile_elems"] = tile_elems
     return t
 
 
-def check_invariants(t, N, conn, blk_off):
-    E = len(conn)
+def check_invariants(t, N, conn_mesh, blk_off):
+    E = len(conn_mesh)
     perm, eo, no = t["perm"], t["elem_off"], t["node_off"]
     T = len(eo) - 1
     assert np.array_equal(np.sort(perm), np.arange(E))
+    # node renumbering: a permutation; tile t's interior nodes are the
+    # contiguous new ids int_off[t]..int_off[t+1] in its first slots, boundary
+    # nodes follow all interior ones
+    nn = t["node_new"]
+    assert np.array_equal(np.sort(nn), np.arange(N))
+    conn = nn[conn_mesh]
+    io = t["int_off"]
+    for ti in range(T):
+        k = io[ti + 1] - io[ti]
+        assert np.array_equal(t["nodes"][no[ti]:no[ti] + k], np.arange(io[ti], io[ti + 1]))
+        assert np.all(t["node_info"][no[ti]:no[ti] + k] & 1) and not np.any(t["node_info"][no[ti] + k:no[ti + 1]] & 1)
+    assert np.array_equal(t["bnd_nodes"], np.arange(io[-1], N))
     # each block's elements stay in the block's range; tiles never straddle blocks
     for b in range(len(blk_off) - 1):
         seg = perm[blk_off[b]:blk_off[b + 1]]
@@ -79,6 +94,7 @@ def check_invariants(t, N, conn, blk_off):
         assert np.array_equal(tn[lc], conn[perm[el]])  # local connectivity maps back to the mesh
         assert np.array_equal(np.unique(conn[perm[el]]), tn)  # slots = the tile's distinct nodes
         col = t["color"][el] & 0x1F
+        assert np.all(np.diff(col.astype(int)) >= 0)  # elements of a tile in colour order
         repeated = np.array([len(np.unique(r)) < 8 for r in conn[perm[el]]])
         assert np.array_equal((t["color"][el] & 0x80) != 0, repeated)  # flag of elements listing a node twice
         assert col.max() < t["num_colors"][ti] <= MAX_COLORS
@@ -107,11 +123,12 @@ def check_invariants(t, N, conn, blk_off):
 
 def tiled_assembly(t, N, felem_internal):
     """Sum element forces the way the kernels do: per tile in colour order,
-    interior nodes complete, the others through partials in ascending tile
-    order.  felem_internal is [E][8][3] in internal (tile) element order."""
+    interior nodes complete, the others through partials stored at slot_dst
+    (by node, ascending tile order).  felem_internal is [E][8][3] in internal
+    (tile) element order."""
     f = np.full((N, 3), np.nan)
     eo, no = t["elem_off"], t["node_off"]
-    partials = np.zeros((len(t["nodes"]), 3))
+    partials = np.full((len(t["bnd_ent"]), 3), np.nan)
     for ti in range(len(eo) - 1):
         acc = np.zeros((no[ti + 1] - no[ti], 3))
         el = np.arange(eo[ti], eo[ti + 1])
@@ -122,13 +139,17 @@ def tiled_assembly(t, N, felem_internal):
         for ls in range(len(acc)):
             s = no[ti] + ls
             if t["node_info"][s] & 1:
+                assert t["slot_dst"][s] == -1
                 f[t["nodes"][s]] = acc[ls]
             else:
-                partials[s] = acc[ls]
+                partials[t["slot_dst"][s]] = acc[ls]
     for b, n in enumerate(t["bnd_nodes"]):
-        ent = t["bnd_ent"][t["bnd_ptr"][b]:t["bnd_ptr"][b + 1]]
+        p0, p1 = t["bnd_ptr"][b], t["bnd_ptr"][b + 1]
+        ent = t["bnd_ent"][p0:p1]
         assert np.all(np.diff(ent) > 0)  # ascending tile order
-        f[n] = partials[ent].sum(axis=0) if len(ent) else 0.0
+        assert np.array_equal(t["slot_dst"][ent], np.arange(p0, p1))
+        assert np.all(t["nodes"][ent] == n)
+        f[n] = partials[p0:p1].sum(axis=0) if p1 > p0 else 0.0
     return f
 
 
@@ -161,7 +182,7 @@ def test_assembly_covers_every_incidence_once(tiles_lib):
     rng = np.random.default_rng(1)
     felem = rng.normal(size=(len(conn), 8, 3))
     ref = np.zeros((len(X), 3))
-    np.add.at(ref, conn.reshape(-1), felem.reshape(-1, 3))
+    np.add.at(ref, t["node_new"][conn].reshape(-1), felem.reshape(-1, 3))
     got = tiled_assembly(t, len(X), felem[t["perm"]])
     np.testing.assert_allclose(got, ref, rtol=0, atol=1e-12)
 
@@ -186,7 +207,7 @@ def test_unstructured_permuted_mesh_with_orphans_and_collapsed_elements(tiles_li
     check_invariants(t, N, conn, blk_off)
     felem = rng.normal(size=(len(conn), 8, 3))
     ref = np.zeros((N, 3))
-    np.add.at(ref, conn.reshape(-1), felem.reshape(-1, 3))
+    np.add.at(ref, t["node_new"][conn].reshape(-1), felem.reshape(-1, 3))
     got = tiled_assembly(t, N, felem[t["perm"]])
     np.testing.assert_allclose(got, ref, rtol=0, atol=1e-12)
 
