@@ -307,19 +307,39 @@ __global__ __launch_bounds__(kBndThreads) void k_boundary_update(TileArgs ta, No
   const int         bi  = (int)g * kBndThreads + threadIdx.x;
   double            rxs = 0.0;
   if (bi < ta.num_bnd) {
-    const int64_t i  = (int64_t)ta.bnd_base + bi;
-    const int     p0 = ta.bnd_ptr[bi], p1 = ta.bnd_ptr[bi + 1];
-    double        f[3] = {0.0, 0.0, 0.0};
-    for (int64_t p = p0; p < p1; p++) {
+    // the node's own state is loaded first, so its latency overlaps the
+    // dependent partial-sum loads
+    const int64_t i     = (int64_t)ta.bnd_base + bi;
+    const int     flags = na.flags[i];
+    double        u[3], v[3], m = 1.0;
+    if (sa.update_v) {
+      load3(na.u, i, u);
+      load3(na.v, i, v);
+      m = na.mass[i];
+    }
+    const int p0 = ta.bnd_ptr[bi], p1 = ta.bnd_ptr[bi + 1];
+    // up to 8 partials are loaded at once, then added in ascending order
+    // (the same order as one at a time); more than 8 continue one by one
+    constexpr int kBatch = 8;
+    double        pv[kBatch][3];
+#pragma unroll
+    for (int i = 0; i < kBatch; i++)
+      if (p0 + i < p1) load3(ta.partials, (int64_t)p0 + i, pv[i]);
+    double f[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+    for (int i = 0; i < kBatch; i++)
+      if (p0 + i < p1) {
+        f[0] += pv[i][0];
+        f[1] += pv[i][1];
+        f[2] += pv[i][2];
+      }
+    for (int64_t p = (int64_t)p0 + kBatch; p < p1; p++) {
       f[0] += ta.partials[3 * p];
       f[1] += ta.partials[3 * p + 1];
       f[2] += ta.partials[3 * p + 2];
     }
-    const int flags = na.flags[i];
     if (rx.coord >= 0 && (flags & kReactionFlag)) rxs = f[rx.coord];
-    double u[3];
-    if (sa.update_v) load3(na.u, i, u);
-    node_step(na, i, flags, f, u, sa);
+    node_update(na, na.bc_table, i, flags, f, u, v, m, sa);
   }
   if (rx.coord < 0) return;
   const double s = block_sum<kBndThreads>(rxs, red);
@@ -336,21 +356,29 @@ void launch_element_tiled(const BlockTable<float>& bt, int64_t E_total, const Ti
 {
   if (t_count <= 0) return;
   const unsigned grid = (unsigned)std::min(t_count, std::max(ta.resident_groups, 1));
-  if (ta.tile_elems == 128)
-    hipLaunchKernelGGL(k_element_tiled<128>, dim3(grid), dim3(128), 0, s, bt, E_total, ta, t_begin, t_count, X, na,
-                       Fip, sip, sa, rx, status);
-  else
-    hipLaunchKernelGGL(k_element_tiled<256>, dim3(grid), dim3(256), 0, s, bt, E_total, ta, t_begin, t_count, X, na,
-                       Fip, sip, sa, rx, status);
+  switch (ta.tile_elems) {
+    case 64:
+      hipLaunchKernelGGL(k_element_tiled<64>, dim3(grid), dim3(64), 0, s, bt, E_total, ta, t_begin, t_count, X, na,
+                         Fip, sip, sa, rx, status);
+      break;
+    case 128:
+      hipLaunchKernelGGL(k_element_tiled<128>, dim3(grid), dim3(128), 0, s, bt, E_total, ta, t_begin, t_count, X,
+                         na, Fip, sip, sa, rx, status);
+      break;
+    default:
+      hipLaunchKernelGGL(k_element_tiled<256>, dim3(grid), dim3(256), 0, s, bt, E_total, ta, t_begin, t_count, X,
+                         na, Fip, sip, sa, rx, status);
+  }
 }
 
 int element_tiled_groups_per_cu(int tile_elems)
 {
   int n = 0;
-  if (tile_elems == 128)
-    (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_element_tiled<128>, 128, 0);
-  else
-    (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_element_tiled<256>, 256, 0);
+  switch (tile_elems) {
+    case 64: (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_element_tiled<64>, 64, 0); break;
+    case 128: (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_element_tiled<128>, 128, 0); break;
+    default: (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_element_tiled<256>, 256, 0);
+  }
   return n;
 }
 

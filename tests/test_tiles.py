@@ -158,7 +158,7 @@ def cube(n, seed=5):
     return X, conn
 
 
-@pytest.mark.parametrize("tile_elems,nodes_per_tile", [(256, 9 * 9 * 5), (128, 9 * 5 * 5)])
+@pytest.mark.parametrize("tile_elems,nodes_per_tile", [(256, 9 * 9 * 5), (128, 9 * 5 * 5), (64, 5 * 5 * 5)])
 def test_structured_cube_tiles_are_boxes(tiles_lib, tile_elems, nodes_per_tile):
     n = 16
     X, conn = cube(n)
