@@ -2,8 +2,9 @@
 
 FP64 translation units (element kernels, node kernels, host driver) are
 compiled with -ffp-contract=off so that every FP64 expression keeps
-NimbleSM's rounding; the mixed-precision element kernel is compiled with
-contraction on.  Packed FP32 instructions are disabled for all device code.
+NimbleSM's rounding; the mixed-precision kernels are compiled with
+contraction on except where a `#pragma clang fp contract(off)` (the FP64
+node update) says otherwise.  Packed FP32 instructions are disabled for all device code.
 """
 from __future__ import annotations
 
@@ -27,8 +28,11 @@ COMMON = [
 SOURCES = {
     "nsm_fp64.hip": ["-ffp-contract=off"],
     "nsm_node.hip": ["-ffp-contract=off"],
-    "nsm_mixed.hip": ["-ffp-contract=fast"],
-    "nsm_tiled.hip": ["-ffp-contract=fast"],
+    # fast-honor-pragmas: FP32 element arithmetic contracts, while the FP64
+    # node update keeps separate multiply and add through its
+    # `#pragma clang fp contract(off)` (plain `fast` ignores that pragma)
+    "nsm_mixed.hip": ["-ffp-contract=fast-honor-pragmas"],
+    "nsm_tiled.hip": ["-ffp-contract=fast-honor-pragmas"],
     "nsm_tiles.cpp": ["-ffp-contract=off", "-x", "hip"],
     "nsm_model.cpp": ["-ffp-contract=off", "-x", "hip"],
 }

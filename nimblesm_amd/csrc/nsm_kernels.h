@@ -56,6 +56,15 @@ struct ReactionArgs {
   double* partials;  // [node tiles]
 };
 
+// A previous step's reaction total, summed by workgroup 0 of the next launch
+// instead of a separate finalize launch (n = 0: none).
+struct ReactionFold {
+  const double* partials;
+  const int*    tiles;
+  int           n;
+  double*       out;
+};
+
 // Workgroup b of nb -> tile index so that XCD (b mod 8) processes one
 // contiguous range of tiles (workgroups are dealt to the 8 XCDs round-robin).
 __device__ __forceinline__ unsigned xcd_tile(unsigned b, unsigned nb)

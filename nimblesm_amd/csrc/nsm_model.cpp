@@ -140,7 +140,9 @@ struct nsm_model {
   int       reaction_coord = 2;
   double*   reaction_hist = nullptr;
   int64_t   reaction_cap = 0;
-  double*   reaction_partials = nullptr;  // one per workgroup of the kernels that sum reactions
+  double*   reaction_partials = nullptr;  // 2 x reaction_stride: one per workgroup of the kernels that
+                                          // sum reactions, for even and for odd steps
+  int64_t   reaction_stride = 0;
   int*      reaction_tiles = nullptr;     // partials holding reaction nodes, ascending
   int       num_reaction_tiles = 0;
   size_t    device_bytes = 0;
@@ -372,13 +374,25 @@ int launch_elements(nsm_model* md)
   return NSM_OK;
 }
 
+// Reaction partials of `step`: two buffers used by alternate steps, so a
+// step's partials can still be summed while the next step writes its own.
+double* reaction_partials_of(const nsm_model* md, int64_t step)
+{
+  return md->reaction_partials + (step & 1) * md->reaction_stride;
+}
+
+bool wants_reaction(const nsm_model* md, int64_t step)
+{
+  return md->num_reaction_tiles > 0 && step >= 0 && step < md->reaction_cap;
+}
+
 // Reaction sum of `step` (if a reaction set exists and the step is recorded):
 // the arguments the kernels fill their partials with.
 bool reaction_args(const nsm_model* md, int64_t step, nsm::ReactionArgs* rx)
 {
   *rx                = nsm::ReactionArgs{-1, nullptr};
-  const bool want_rx = md->num_reaction_tiles > 0 && step >= 0 && step < md->reaction_cap;
-  if (want_rx) *rx = nsm::ReactionArgs{md->reaction_coord, md->reaction_partials};
+  const bool want_rx = wants_reaction(md, step);
+  if (want_rx) *rx = nsm::ReactionArgs{md->reaction_coord, reaction_partials_of(md, step)};
   return want_rx;
 }
 
@@ -387,7 +401,7 @@ int launch_reaction(nsm_model* md, int64_t step)
   size_t ti;
   int    st = time_begin(md, NSM_KERNEL_REACTION, &ti);
   if (st) return st;
-  nsm::launch_reaction_finalize(md->reaction_partials, md->reaction_tiles, md->num_reaction_tiles,
+  nsm::launch_reaction_finalize(reaction_partials_of(md, step), md->reaction_tiles, md->num_reaction_tiles,
                                 md->reaction_hist + step, md->stream);
   NSM_LAUNCHED();
   return time_end(md, ti);
@@ -413,35 +427,47 @@ int launch_node(nsm_model* md, const nsm::StepArgs& sa, int64_t step)
 }
 
 // Tile-local step: element tiles (with the interior-node update), then the
-// boundary-node update, then the reaction sum.
-int launch_tiled(nsm_model* md, const nsm::StepArgs& sa, int64_t step)
+// boundary-node update.  The reaction total of `step` is left to the caller
+// (`*pending` = step when it has one); the total of `fold_step` (a previous
+// step, or -1) is summed by the first element launch.
+int launch_tiled(nsm_model* md, const nsm::StepArgs& sa, int64_t step, int64_t fold_step, int64_t* pending)
 {
   nsm::ReactionArgs     rx;
   const bool            want_rx = reaction_args(md, step, &rx);
   const nsm::NodeArrays na      = md->node_arrays();
-  int                   st      = NSM_OK;
+  nsm::ReactionFold     fold{nullptr, nullptr, 0, nullptr};
+  if (fold_step >= 0)
+    fold = nsm::ReactionFold{reaction_partials_of(md, fold_step), md->reaction_tiles, md->num_reaction_tiles,
+                             md->reaction_hist + fold_step};
+  int st = NSM_OK;
   for (int b0 = 0; b0 < md->num_blocks; b0 += nsm::kMaxLaunchBlocks) {
     const int nb = std::min(nsm::kMaxLaunchBlocks, md->num_blocks - b0);
     const int t0 = md->blk_tile_off[(size_t)b0], t1 = md->blk_tile_off[(size_t)(b0 + nb)];
-    size_t    ti;
+    if (t1 <= t0) continue;  // no launch: the fold goes to the next one
+    size_t ti;
     if ((st = time_begin(md, NSM_KERNEL_ELEMENT, &ti))) return st;
     nsm::launch_element_tiled(float_table(md, b0, nb), md->E, md->tile_args, t0, t1 - t0, md->X, na,
-                              (float*)md->Fip, (float*)md->sip, sa, rx, md->status, md->stream);
+                              (float*)md->Fip, (float*)md->sip, sa, rx, fold, md->status, md->stream);
     NSM_LAUNCHED();
     if ((st = time_end(md, ti))) return st;
+    fold.n = 0;
   }
   size_t ti;
   if ((st = time_begin(md, NSM_KERNEL_NODE, &ti))) return st;
   nsm::launch_boundary_update(md->tile_args, na, sa, rx, md->stream);
   NSM_LAUNCHED();
   if ((st = time_end(md, ti))) return st;
-  return want_rx ? launch_reaction(md, step) : NSM_OK;
+  *pending = want_rx ? step : -1;
+  return NSM_OK;
 }
 
 // One step's force evaluation and node update (update_v = 0: f_int only).
-int launch_step(nsm_model* md, const nsm::StepArgs& sa, int64_t step)
+// The slot path finalizes the reaction of `step` itself; the tiled path
+// leaves it pending (see launch_tiled) and folds `fold_step` instead.
+int launch_step(nsm_model* md, const nsm::StepArgs& sa, int64_t step, int64_t fold_step, int64_t* pending)
 {
-  if (md->tiled) return launch_tiled(md, sa, step);
+  *pending = -1;
+  if (md->tiled) return launch_tiled(md, sa, step, fold_step, pending);
   int st = launch_elements(md);
   return st ? st : launch_node(md, sa, step);
 }
@@ -707,6 +733,7 @@ int nsm_model_create(int64_t num_nodes, const double* reference_coordinates, int
   const size_t num_partials = md->tiled ? (size_t)md->tiles.num_tiles +
                                               nsm::boundary_grid((int)md->tiles.bnd_nodes.size())
                                         : (size_t)nsm::node_grid(num_nodes);
+  md->reaction_stride = (int64_t)std::max<size_t>(num_partials, 1);
   if ((st = dalloc(&md->X, N3, &tot)) || (st = dalloc(&md->u, N3, &tot)) || (st = dalloc(&md->v, N3, &tot)) ||
       (st = dalloc(&md->a, N3, &tot)) || (st = dalloc(&md->f, N3, &tot)) ||
       (st = dalloc(&md->m, (size_t)num_nodes, &tot)) || (st = dalloc(&md->conn, (size_t)(8 * E), &tot)) ||
@@ -715,7 +742,7 @@ int nsm_model_create(int64_t num_nodes, const double* reference_coordinates, int
       (st = dalloc((char**)&md->sip, (size_t)(48 * E) * ipw, &tot)) ||
       (st = dalloc((char**)&md->slots, num_slots * ipw, &tot)) ||
       (st = dalloc(&md->status, 1, &tot)) || (st = dalloc(&md->flags, (size_t)num_nodes, &tot)) ||
-      (st = dalloc(&md->reaction_partials, std::max<size_t>(num_partials, 1), &tot)) ||
+      (st = dalloc(&md->reaction_partials, 2 * std::max<size_t>(num_partials, 1), &tot)) ||
       (md->tiled && (st = setup_tiles(md))))
     return cleanup(st);
 
@@ -905,7 +932,8 @@ int nsm_model_compute_internal_force(nsm_model* md, double time_previous, double
   if (st) return st;
   const nsm::StepTimes none{0.0, 0.0, 0.0, 0.0};
   const nsm::StepArgs  sa{none, none, 0, 0, 0, 0};
-  if ((st = launch_step(md, sa, -1))) return st;
+  int64_t              pending = -1;
+  if ((st = launch_step(md, sa, -1, -1, &pending))) return st;
   return check_status(md);
 }
 
@@ -975,18 +1003,21 @@ int nsm_model_explicit_steps(nsm_model* md, int64_t step_begin, int64_t step_end
   nsm::StepTimes cur = step_times(&t, dt_user);
   nsm::launch_prekick(md->node_arrays(), cur, md->stream);
   NSM_LAUNCHED();
+  int64_t pending = -1;  // step whose reaction total is not yet summed
   for (int64_t step = step_begin; step < step_end; step++) {
     const bool     last = step + 1 == step_end;
     double         tn   = t;
     nsm::StepTimes nxt  = cur;
     if (!last) nxt = step_times(&tn, dt_user);
     const nsm::StepArgs sa{cur, nxt, (int)is_output(step, num_load_steps, output_frequency), (int)!last, (int)last, 1};
-    if ((st = launch_step(md, sa, step))) return st;
+    // the previous step's pending reaction total is summed inside this launch
+    if ((st = launch_step(md, sa, step, pending, &pending))) return st;
     if (!last) {
       t   = tn;
       cur = nxt;
     }
   }
+  if (pending >= 0 && (st = launch_reaction(md, pending))) return st;
   *time_current = t;
   return check_status(md);
 }

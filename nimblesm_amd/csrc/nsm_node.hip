@@ -113,16 +113,9 @@ __global__ __launch_bounds__(kNodeThreads) void k_reaction_finalize(const double
                                                                     const int* __restrict__ tiles, int num_tiles,
                                                                     double* __restrict__ out)
 {
-  __shared__ double red[kNodeThreads];
-  double            s = 0.0;
-  for (int k = threadIdx.x; k < num_tiles; k += kNodeThreads) s += partials[tiles[k]];
-  red[threadIdx.x] = s;
-  __syncthreads();
-  for (int w = kNodeThreads / 2; w > 0; w >>= 1) {
-    if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) *out = -red[0];
+  __shared__ double red[256];
+  const double      total = reaction_total<kNodeThreads>(partials, tiles, num_tiles, red);
+  if (threadIdx.x == 0) *out = total;
 }
 
 __global__ __launch_bounds__(kNodeThreads) void k_mass_gather(int64_t N, const uint32_t* __restrict__ node_ptr,

@@ -91,6 +91,28 @@ __device__ __forceinline__ void node_update(const NodeArrays& na, const BcEntry*
   if (sa.write_a) store3(na.a, i, a);
 }
 
+// Reaction total -sum_k partials[tiles[k]] of one step, summed in 256
+// interleaved lanes (lane v takes k = v, v + 256, ...) and then a fixed tree,
+// so the result is the same for any workgroup size kThreads dividing 256.
+// red: 256 doubles of LDS.  Valid in thread 0.
+template <int kThreads>
+__device__ __forceinline__ double reaction_total(const double* partials, const int* tiles, int n, double* red)
+{
+  constexpr int kLanes = 256;
+  static_assert(kLanes % kThreads == 0, "workgroup size must divide 256");
+  for (int v = threadIdx.x; v < kLanes; v += kThreads) {
+    double s = 0.0;
+    for (int k = v; k < n; k += kLanes) s += partials[tiles[k]];
+    red[v] = s;
+  }
+  __syncthreads();
+  for (int w = kLanes / 2; w > 0; w >>= 1) {
+    for (int v = threadIdx.x; v < w; v += kThreads) red[v] += red[v + w];
+    __syncthreads();
+  }
+  return -red[0];
+}
+
 // node_update with v and m loaded from the node arrays.
 __device__ __forceinline__ void node_step(const NodeArrays& na, int64_t i, int flags, const double f[3],
                                           double u[3], const StepArgs& sa)

@@ -33,9 +33,10 @@ unsigned      boundary_grid(int num_bnd);
 // Tiles [t_begin, t_begin + t_count) (all inside the blocks of bt): element
 // arithmetic, force assembly in LDS, the step of the tile-interior nodes,
 // partials of the others.  Reaction partials go to rx.partials[tile].
+// Workgroup 0 first writes the folded reaction total of a previous step.
 void launch_element_tiled(const BlockTable<float>& bt, int64_t E_total, const TileArgs& ta, int t_begin, int t_count,
                           const double* X, const NodeArrays& na, float* Fip, float* sip, const StepArgs& sa,
-                          ReactionArgs rx, int* status, hipStream_t s);
+                          ReactionArgs rx, ReactionFold fold, int* status, hipStream_t s);
 // Element-kernel workgroups one CU holds at once (occupancy).
 int element_tiled_groups_per_cu(int tile_elems);
 // Step of the remaining nodes from their partials (ascending tile order).

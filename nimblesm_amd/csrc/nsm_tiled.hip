@@ -150,7 +150,8 @@ __global__ __launch_bounds__(kThreads) void k_element_tiled(const BlockTable<flo
                                                             TileArgs ta, int t_begin, int t_count,
                                                             const double* __restrict__ X, NodeArrays na,
                                                             float* __restrict__ Fip, float* __restrict__ sip,
-                                                            StepArgs sa, ReactionArgs rx, int* __restrict__ status)
+                                                            StepArgs sa, ReactionArgs rx, ReactionFold fold,
+                                                            int* __restrict__ status)
 {
   constexpr int kNodes = 2 * kThreads;
   __shared__ double  sX[kNodes][3];
@@ -159,6 +160,12 @@ __global__ __launch_bounds__(kThreads) void k_element_tiled(const BlockTable<flo
   __shared__ double  red[kThreads / 64];
   __shared__ BcEntry sbc[kMaxBcEntries];
   __shared__ int     bad;
+  __shared__ double  fold_red[256];
+
+  if (fold.n > 0 && blockIdx.x == 0) {  // uniform
+    const double total = reaction_total<kThreads>(fold.partials, fold.tiles, fold.n, fold_red);
+    if (threadIdx.x == 0) *fold.out = total;
+  }
 
   // Workgroup w (numbered so that each XCD holds a contiguous run of w)
   // processes tiles t_begin + w, + G, + 2G, ... (G workgroups): at any time
@@ -352,22 +359,22 @@ unsigned boundary_grid(int num_bnd) { return (unsigned)((num_bnd + kBndThreads -
 
 void launch_element_tiled(const BlockTable<float>& bt, int64_t E_total, const TileArgs& ta, int t_begin, int t_count,
                           const double* X, const NodeArrays& na, float* Fip, float* sip, const StepArgs& sa,
-                          ReactionArgs rx, int* status, hipStream_t s)
+                          ReactionArgs rx, ReactionFold fold, int* status, hipStream_t s)
 {
   if (t_count <= 0) return;
   const unsigned grid = (unsigned)std::min(t_count, std::max(ta.resident_groups, 1));
   switch (ta.tile_elems) {
     case 64:
       hipLaunchKernelGGL(k_element_tiled<64>, dim3(grid), dim3(64), 0, s, bt, E_total, ta, t_begin, t_count, X, na,
-                         Fip, sip, sa, rx, status);
+                         Fip, sip, sa, rx, fold, status);
       break;
     case 128:
       hipLaunchKernelGGL(k_element_tiled<128>, dim3(grid), dim3(128), 0, s, bt, E_total, ta, t_begin, t_count, X,
-                         na, Fip, sip, sa, rx, status);
+                         na, Fip, sip, sa, rx, fold, status);
       break;
     default:
       hipLaunchKernelGGL(k_element_tiled<256>, dim3(grid), dim3(256), 0, s, bt, E_total, ta, t_begin, t_count, X,
-                         na, Fip, sip, sa, rx, status);
+                         na, Fip, sip, sa, rx, fold, status);
   }
 }
 

@@ -169,3 +169,21 @@ def test_tiled_matches_slots_over_a_run(hip_device, monkeypatch):
     eR = rel_err(res["tiles"][1], res["slots"][1])
     print(f"tiled vs slots after 200 steps: u {eu:.2e} R {eR:.2e}")
     assert eu <= 1e-7 and eR <= 1e-6
+
+
+def test_reaction_history_independent_of_call_chunking(hip_device):
+    """A step's reaction total is summed inside the next step's element
+    launch, or by a separate launch after a call's last step; both use the
+    same fixed order, so chunking the run differently changes no bit."""
+    prob = N.CubeProblem(n=12, materials=[N.problems.STEEL, N.problems.ALUMINIUM], num_steps=40)
+    hist = []
+    for chunks in ([40], [1, 7, 32], [13, 1, 1, 25]):
+        m = prob.make_model("mixed")
+        integ = prob.make_integrator(m)
+        for c in chunks:
+            integ.Advance(c)
+        hist.append((m.GetReactionHistory(0, 40), m.GetVectorNodeData("displacement")))
+        m.close()
+    for R, u in hist[1:]:
+        assert np.array_equal(R, hist[0][0]) and np.array_equal(u, hist[0][1])
+    assert np.all(hist[0][0][1:] != 0.0)
