@@ -157,6 +157,12 @@ int nsm_model_update_states(nsm_model* model);
  * the quantity of interest of the synthetic benchmarks. */
 int nsm_model_set_reaction_nodes(nsm_model* model, int64_t num_nodes, const int* node_ids, int coordinate);
 int nsm_model_get_reaction_history(nsm_model* model, int64_t first_step, int64_t count, double* out);
+/* The quantity of interest of the synthetic workloads: the reaction history
+ * of steps [0, num_steps) sampled on the device at num_samples equally
+ * spaced times.  Sample k is taken at p = (k+1)*num_steps/num_samples steps,
+ * linearly interpolated between integer steps j (value after step j-1;
+ * step 0 is the model at rest, value 0).  out: num_samples doubles (host). */
+int nsm_model_sample_reaction_history(nsm_model* model, int64_t num_steps, int num_samples, double* out);
 
 /* Steps [step_begin, step_end) of ExplicitTimeIntegrator::Integrate with
  * t += dt_user, delta = t - t_prev; *time_current is read and updated.  All

@@ -48,6 +48,7 @@ SIGNATURES = {
     "nsm_model_update_states": (_I, [_P]),
     "nsm_model_set_reaction_nodes": (_I, [_P, _I64, _P, _I]),
     "nsm_model_get_reaction_history": (_I, [_P, _I64, _I64, _P]),
+    "nsm_model_sample_reaction_history": (_I, [_P, _I64, _I, _P]),
     "nsm_model_explicit_steps": (_I, [_P, _I64, _I64, _I64, _I64, _D, C.POINTER(_D)]),
     "nsm_model_get_field": (_I, [_P, _I, _P]),
     "nsm_model_set_field": (_I, [_P, _I, _P]),

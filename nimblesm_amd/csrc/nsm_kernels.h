@@ -130,6 +130,8 @@ struct StepArgs;
 // optional per-tile reaction partials.
 template <typename SlotT>
 void launch_node_update(const NodeArrays& na, const SlotT* slots, const StepArgs& sa, ReactionArgs rx, hipStream_t s);
+// reaction history sampled at num_samples equally spaced times (device out)
+void launch_sample_history(const double* hist, int64_t n, int num_samples, double* out, hipStream_t s);
 // out = -sum_k partials[tiles[k]], k ascending (fixed tree, reproducible)
 void launch_reaction_finalize(const double* partials, const int* tiles, int num_tiles, double* out, hipStream_t s);
 

@@ -975,6 +975,28 @@ int nsm_model_get_reaction_history(nsm_model* md, int64_t first_step, int64_t co
   return NSM_OK;
 }
 
+int nsm_model_sample_reaction_history(nsm_model* md, int64_t num_steps, int num_samples, double* out)
+{
+  if (!md || (num_samples > 0 && !out)) return fail(NSM_ERR_ARG, "null argument");
+  if (num_steps < 1 || num_steps > md->reaction_cap)
+    return fail(NSM_ERR_ARG, "reaction history of %lld steps not recorded", (long long)num_steps);
+  if (num_samples < 0) return fail(NSM_ERR_ARG, "negative sample count");
+  if (num_samples == 0) return NSM_OK;
+  int st = ensure_device(md);
+  if (st) return st;
+  double* d_out = nullptr;
+  size_t  tmp   = 0;
+  if ((st = dalloc(&d_out, (size_t)num_samples, &tmp))) return st;
+  nsm::launch_sample_history(md->reaction_hist, num_steps, num_samples, d_out, md->stream);
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess) e = hipMemcpyAsync(out, d_out, (size_t)num_samples * sizeof(double), hipMemcpyDeviceToHost,
+                                          md->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(md->stream);
+  free_dev(d_out);
+  if (e != hipSuccess) return fail(NSM_ERR_HIP, "sampling the reaction history: %s", hipGetErrorString(e));
+  return NSM_OK;
+}
+
 int nsm_model_explicit_steps(nsm_model* md, int64_t step_begin, int64_t step_end, int64_t num_load_steps,
                              int64_t output_frequency, double dt_user, double* time_current)
 {

@@ -130,7 +130,32 @@ __global__ __launch_bounds__(kNodeThreads) void k_mass_gather(int64_t N, const u
   m[i] = s;
 }
 
+// Reaction history sampled at num_samples equally spaced times: sample k at
+// p = (k+1) n / num_samples steps, linear between integer steps, where step
+// j >= 1 holds hist[j-1] (the total after step j-1) and step 0 is 0 (a model
+// at rest).
+__global__ __launch_bounds__(kNodeThreads) void k_sample_history(const double* __restrict__ hist, int64_t n,
+                                                                 int num_samples, double* __restrict__ out)
+{
+  const int k = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  if (k >= num_samples) return;
+  const double  p  = (double)(k + 1) * (double)n / (double)num_samples;
+  int64_t       j  = (int64_t)p;  // p >= 0
+  if (j > n) j = n;
+  const double  w  = p - (double)j;
+  const double  r0 = j == 0 ? 0.0 : hist[j - 1];
+  const double  r1 = j < n ? hist[j] : r0;
+  out[k]           = (1.0 - w) * r0 + w * r1;
+}
+
 }  // namespace
+
+void launch_sample_history(const double* hist, int64_t n, int num_samples, double* out, hipStream_t s)
+{
+  if (num_samples <= 0) return;
+  hipLaunchKernelGGL(k_sample_history, dim3((unsigned)((num_samples + kNodeThreads - 1) / kNodeThreads)),
+                     dim3(kNodeThreads), 0, s, hist, n, num_samples, out);
+}
 
 unsigned node_grid(int64_t num_nodes) { return (unsigned)((num_nodes + kNodeThreads - 1) / kNodeThreads); }
 

@@ -244,6 +244,14 @@ class ModelData:
         check(lib().nsm_model_get_reaction_history(self._h, first_step, count, out.ctypes.data))
         return out
 
+    def SampleReactionHistory(self, num_steps: int, num_samples: int = 64) -> np.ndarray:
+        """Reaction history of steps [0, num_steps) at num_samples equally
+        spaced times (sample k at (k+1)*num_steps/num_samples steps, linear
+        between steps; value 0 at rest before step 0)."""
+        out = np.zeros(num_samples)
+        check(lib().nsm_model_sample_reaction_history(self._h, int(num_steps), int(num_samples), out.ctypes.data))
+        return out
+
     # -- instrumentation
     def Synchronize(self):
         check(lib().nsm_model_synchronize(self._h))
@@ -286,7 +294,7 @@ class ExplicitTimeIntegrator:
         m.ComputeLumpedMass()
         self.critical_time_step = m.GetCriticalTimeStep()
         m.ApplyInitialConditions()
-        m.ApplyKinematicConditions(0.0, 0.0)
+        m.ApplyKinematicConditions(self.time_current, self.time_current)
         return self
 
     def is_output_step(self, step: int) -> bool:

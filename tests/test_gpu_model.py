@@ -213,3 +213,19 @@ def test_reaction_over_scattered_nodes(hip_device):
         scale = np.abs(run.reaction).max()
         assert np.abs(R - run.reaction).max() <= tol * scale, (precision, np.abs(R - run.reaction).max() / scale)
         m.close()
+
+
+def test_sampled_reaction_history(hip_device):
+    """64-point QoI on the device: identical to the oracle's sampling rule
+    applied to the device history; FP64 history sampled = oracle's sampled
+    history to 1e-13, mixed to 1e-5."""
+    prob = N.CubeProblem(n=10, materials=[N.problems.STEEL, N.problems.ALUMINIUM], num_steps=100)
+    run, _ = run_oracle_problem(prob, prob.num_steps)
+    ref = O.sample_history(run.reaction, 64)
+    for precision, tol in (("fp64", 1e-13), ("mixed", 1e-5)):
+        m, integ = run_gpu_problem(prob, prob.num_steps, precision)
+        R = m.GetReactionHistory(0, prob.num_steps)
+        s = m.SampleReactionHistory(prob.num_steps, 64)
+        assert np.array_equal(s, O.sample_history(R, 64))
+        assert max_rel_err(s, ref) <= tol, (precision, max_rel_err(s, ref))
+        m.close()
