@@ -8,13 +8,19 @@ CSR node gather), and the top-face reaction sum.  Inputs are resident in HBM
 before timing starts.
 
     python bench.py [--gpus 1] [--steps 500] [--warmup 20] [--precision mixed|fp64]
-                    [--workload cube128_2blk] [--cpu-steps 8] [--no-cpu]
+                    [--workload cube128_2blk] [--cpu-steps 8] [--no-cpu] [--uq NE,NA]
 
 Prints one JSON line.  `value` is element-steps/s of the timed K steps.
 `roofline` covers the dominant (element) kernel: its algorithmic bytes per
 launch (algorithmic_bytes below) over its average launch duration from HIP
 events on the model's stream.  `cpu_baseline` times the FP64 CPU oracle (OpenMP, deterministic
 gather) on a bounded sample of the same workload.
+
+--uq NE,NA adds an off-nominal ensemble (NE exact samples with the bulk and
+shear moduli of every block scaled by 1 + 0.2 x_k, x_k evenly spaced in
+[-1, 1], and NA approximate samples with closure coefficients interpolating
+between neighbouring exact samples) and reports the closure kernel in a "uq"
+object; the headline line is the default run without it.
 """
 from __future__ import annotations
 
@@ -63,6 +69,34 @@ def algorithmic_bytes(prob, s_bytes, assembly):
     return per_launch, step
 
 
+def closure_bytes(num_nodes, ne, na):
+    """Algorithmic bytes of one closure launch: per node the nominal and NE
+    exact forces (24 B each), flags and mass (9 B), and per approximate
+    sample u, v read and f, u, v written (120 B)."""
+    return num_nodes * (33 + 24 * ne + 120 * na)
+
+
+def uq_ensemble(prob, ne, na):
+    """Moduli [ne, blocks] and closure coefficients [na, ne + 1] of --uq."""
+    K0 = np.array([b.material.bulk_modulus for b in prob.blocks()])
+    G0 = np.array([b.material.shear_modulus for b in prob.blocks()])
+    x = np.linspace(-1.0, 1.0, ne) if ne > 1 else np.zeros(ne)
+    K = K0[None, :] * (1 + 0.2 * x[:, None])
+    G = G0[None, :] * (1 + 0.2 * x[:, None])
+    c = np.zeros((na, ne + 1))
+    for s in range(na):
+        if ne == 0:
+            c[s, 0] = 1.0
+            continue
+        p = s * (ne - 1) / max(na - 1, 1)  # position among the exact samples
+        k = min(int(p), ne - 1)
+        w = p - k
+        c[s, 1 + k] += 1.0 - w
+        if w > 0:
+            c[s, 2 + k] += w
+    return K, G, c
+
+
 def load_traffic(workload, precision):
     """HBM bytes per element-kernel launch measured with rocprofv3 --pmc
     (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel calibration), as
@@ -106,6 +140,7 @@ def main():
     ap.add_argument("--workload", default="cube128_2blk")
     ap.add_argument("--cpu-steps", type=int, default=8)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--uq", default="", metavar="NE,NA", help="add NE exact and NA approximate UQ samples")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run 5 device copies of MIB MiB (calibrates rocprofv3 FETCH_SIZE / WRITE_SIZE)")
     args = ap.parse_args()
@@ -138,6 +173,11 @@ def main():
     s_bytes = 4 if args.precision == "mixed" else 8
     total = args.warmup + args.steps
     model = prob.make_model(args.precision)
+    ne = na = 0
+    if args.uq:
+        ne, na = (int(x) for x in args.uq.split(","))
+        uq = uq_ensemble(prob, ne, na)
+        model.SetUqSamples(*uq)
 
     # warm-up run, then restart from rest for the timed run of K steps
     integ = prob.make_integrator(model, num_steps=max(total, 1))
@@ -146,6 +186,8 @@ def main():
     zeros = np.zeros((prob.num_nodes, 3))
     for label in ("displacement", "velocity", "acceleration", "internal_force"):
         model.SetVectorNodeData(label, zeros)
+    if args.uq:
+        model.SetUqSamples(*uq)  # every sample back at rest
     integ = prob.make_integrator(model, num_steps=args.steps)
 
     model.EnableKernelTiming(True)
@@ -160,6 +202,7 @@ def main():
     el_ms, el_n = model.KernelTime("element")
     nd_ms, nd_n = model.KernelTime("node")
     rx_ms, rx_n = model.KernelTime("reaction")
+    cl_ms, cl_n = model.KernelTime("closure")
     model.EnableKernelTiming(False)
 
     assembly = model.AssemblyInfo()
@@ -218,6 +261,16 @@ def main():
         },
         "reaction_final": float(R[-1]),
     }
+    if args.uq:
+        cb = closure_bytes(prob.num_nodes, ne, na)
+        out["config"]["uq"] = {"exact_samples": ne, "approx_samples": na}
+        out["uq"] = {
+            "trajectories": 1 + ne + na,
+            "trajectory_steps_per_s": (1 + ne + na) * args.steps / elapsed,
+            "closure_ms_avg": cl_ms, "closure_launches": cl_n, "closure_alg_bytes_per_launch": cb,
+            "closure_achieved_GBs": cb / (cl_ms * 1e-3) / 1e9 if cl_ms > 0 else 0.0,
+            "closure_frac": cb / (cl_ms * 1e-3) / 1e9 / HBM_PEAK_GBS if cl_ms > 0 else 0.0,
+        }
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(prob, args.cpu_steps)
     print(json.dumps(out), flush=True)

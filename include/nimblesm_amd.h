@@ -64,7 +64,8 @@ typedef enum nsm_kernel {
                              (tiled: also the update of tile-interior nodes)   */
   NSM_KERNEL_NODE    = 1, /* CSR gather + central-difference node kernel
                              (tiled: update of nodes shared between tiles)     */
-  NSM_KERNEL_REACTION = 2 /* reaction-force sum                                  */
+  NSM_KERNEL_REACTION = 2, /* reaction-force sum                                 */
+  NSM_KERNEL_CLOSURE  = 3  /* UQ closure + update of the approximate samples      */
 } nsm_kernel;
 
 const char* nsm_status_string(int status);
@@ -211,6 +212,35 @@ int nsm_model_kernel_time(nsm_model* model, int kernel, double* avg_ms, int64_t*
 /* Sizes of the model: nodes, elements, CSR entries, bytes of device memory. */
 int nsm_model_info(nsm_model* model, int64_t* num_nodes, int64_t* num_elem, int64_t* csr_entries,
                    int64_t* device_bytes);
+
+/* ---------------------------------------------------------------------------
+ * Off-nominal ensemble: the reference's UQ model (src/uq/nimble_uq.cc:145-345,
+ * src/uq/nimble_uq_model_data.cc:145-248, src/uq/nimble_uq_block.cc:57-164).
+ * Beside the nominal trajectory the model advances num_exact "exact" samples,
+ * each with its own bulk and shear modulus per block (the off-nominal stress
+ * of NeohookeanMaterial::GetOffNominalStress, src/nimble_material.cc:312-342),
+ * and num_approx "approximate" samples whose internal force is the closure
+ * (UqModel::ApplyClosure, src/uq/nimble_uq.cc:327-346)
+ *     f_s = c[s][0] f_nominal + sum_k c[s][k+1] f_exact_k.
+ * All samples share the lumped mass; initial and prescribed velocities apply
+ * to every sample, prescribed displacements only to the nominal trajectory
+ * (src/nimble_boundary_condition_manager.h:93-217).  Samples start at rest.
+ * Samples are numbered exact first, then approximate (UqModel::Forces()).
+ * ------------------------------------------------------------------------- */
+
+/* Replaces UqModel::Initialize / Allocate / Setup (src/uq/nimble_uq.cc:270-325)
+ * with the parameters already scaled (UqModel::ScaleParameters, :220-244):
+ * bulk_modulus, shear_modulus [num_exact][num_blocks] in the block order of
+ * nsm_model_create; coefficients [num_approx][num_exact + 1].  Replaces any
+ * previous ensemble and zeroes every sample's state; 0, 0 removes it. */
+int nsm_model_set_uq_samples(nsm_model* model, int num_exact, const double* bulk_modulus,
+                             const double* shear_modulus, int num_approx, const double* coefficients);
+/* Host copies of a sample's node fields (NSM_FIELD_DISPLACEMENT, _VELOCITY,
+ * _ACCELERATION, _INTERNAL_FORCE) in mesh node order: the reference's
+ * exact_* / sample_* node data (src/uq/nimble_uq.cc:281-300).  The reference
+ * keeps no sample acceleration; here it is (1/m) f_s of the last step. */
+int nsm_model_get_uq_field(nsm_model* model, int sample, int field, double* out);
+int nsm_model_set_uq_field(nsm_model* model, int sample, int field, const double* in);
 
 /* ---------------------------------------------------------------------------
  * Synthetic input: n^3 HEX8 unit cube, node (i,j,k) -> i+(n+1)(j+(n+1)k),

@@ -24,7 +24,7 @@ FIELDS = {
     "stress": 7,
 }
 BC_KINDS = {"initial_velocity": 0, "prescribed_velocity": 1, "prescribed_displacement": 2}
-KERNELS = {"element": 0, "node": 1, "reaction": 2}
+KERNELS = {"element": 0, "node": 1, "reaction": 2, "closure": 3}
 
 # every exported symbol with (restype, argtypes)
 _P, _I64, _I, _D = C.c_void_p, C.c_int64, C.c_int, C.c_double
@@ -53,6 +53,9 @@ SIGNATURES = {
     "nsm_model_get_field": (_I, [_P, _I, _P]),
     "nsm_model_set_field": (_I, [_P, _I, _P]),
     "nsm_model_field_device_pointer": (_I, [_P, _I, C.POINTER(_P), C.POINTER(_I)]),
+    "nsm_model_set_uq_samples": (_I, [_P, _I, _P, _P, _I, _P]),
+    "nsm_model_get_uq_field": (_I, [_P, _I, _I, _P]),
+    "nsm_model_set_uq_field": (_I, [_P, _I, _I, _P]),
     "nsm_model_synchronize": (_I, [_P]),
     "nsm_model_enable_kernel_timing": (_I, [_P, _I]),
     "nsm_model_kernel_time": (_I, [_P, _I, C.POINTER(_D), C.POINTER(_I64)]),

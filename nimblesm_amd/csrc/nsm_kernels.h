@@ -130,6 +130,20 @@ struct StepArgs;
 // optional per-tile reaction partials.
 template <typename SlotT>
 void launch_node_update(const NodeArrays& na, const SlotT* slots, const StepArgs& sa, ReactionArgs rx, hipStream_t s);
+// Off-nominal ensemble: sample s owns u, v, a, f at state + (4 s + j) stride,
+// j = 0..3 (exact samples first).  The closure kernel forms each approximate
+// sample's force f_s = c[s][0] f_nominal + sum_k c[s][k+1] f_exact_k
+// (src/uq/nimble_uq.cc:327-346, same operation order) and applies the node
+// update of node_step to it; na carries the samples' mass, flags and
+// kinematic-condition table.
+struct UqClosure {
+  int           num_exact, num_approx;
+  int64_t       stride;     // 3 N
+  const double* f_nominal;  // [N][3]
+  double*       state;
+  const double* coeff;      // [num_approx][num_exact + 1]
+};
+void launch_uq_closure(const NodeArrays& na, const UqClosure& uc, const StepArgs& sa, hipStream_t s);
 // reaction history sampled at num_samples equally spaced times (device out)
 void launch_sample_history(const double* hist, int64_t n, int num_samples, double* out, hipStream_t s);
 // out = -sum_k partials[tiles[k]], k ascending (fixed tree, reproducible)

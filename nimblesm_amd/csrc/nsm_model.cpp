@@ -155,6 +155,14 @@ struct nsm_model {
   nsm::TileArgs        tile_args{};
   std::vector<void*>   tile_bufs;
 
+  // off-nominal ensemble (UQ): uq_exact exact samples, then uq_approx
+  // approximate ones; sample s owns u, v, a, f at uq_state + (4 s + j) 3N
+  int                 uq_exact = 0, uq_approx = 0;
+  std::vector<double> uq_K, uq_G;           // [uq_exact][num_blocks]
+  double*             uq_state    = nullptr;
+  double*             uq_coeff    = nullptr;  // [uq_approx][uq_exact + 1]
+  nsm::BcEntry*       uq_bc_table = nullptr;  // bc_table without prescribed displacements
+
   std::vector<BcRecord> bcs;
   std::vector<uint8_t>  reaction_mask;  // host copy of the reaction bit per node
   bool                  bc_dirty = true;
@@ -167,7 +175,12 @@ struct nsm_model {
   size_t                              event_next = 0;
   std::vector<std::pair<int, size_t>> timed;  // (kernel, event index)
 
-  nsm::NodeArrays node_arrays() const
+  int uq_samples() const { return uq_exact + uq_approx; }
+
+  // node arrays of the nominal trajectory (sample < 0) or of UQ sample
+  // `sample`, which shares mass and flags but not the kinematic-condition
+  // table
+  nsm::NodeArrays node_arrays(int sample = -1) const
   {
     nsm::NodeArrays na;
     na.num_nodes   = N;
@@ -182,8 +195,21 @@ struct nsm_model {
     na.flags     = flags;
     na.bc_table  = bc_table;
     na.num_bc    = num_bc_entries;
+    if (sample >= 0) {
+      const int64_t N3 = 3 * N;
+      double*       b  = uq_state + 4 * (int64_t)sample * N3;
+      na.u             = b;
+      na.v             = b + N3;
+      na.a             = b + 2 * N3;
+      na.f             = b + 3 * N3;
+      na.bc_table      = uq_bc_table;
+    }
     return na;
   }
+
+  // moduli of block b for the nominal trajectory or an exact sample
+  double bulk(int sample, int b) const { return sample < 0 ? K[b] : uq_K[(size_t)sample * num_blocks + b]; }
+  double shear(int sample, int b) const { return sample < 0 ? G[b] : uq_G[(size_t)sample * num_blocks + b]; }
 };
 
 namespace {
@@ -280,6 +306,23 @@ int rebuild_bcs(nsm_model* md)
   if (st) return st;
   if (!table.empty())
     NSM_HIP(hipMemcpy(md->bc_table, table.data(), table.size() * sizeof(nsm::BcEntry), hipMemcpyHostToDevice));
+  // UQ samples take initial and prescribed velocities but not prescribed
+  // displacements (src/nimble_boundary_condition_manager.h:184-215 sets the
+  // nominal velocity only)
+  free_dev(md->uq_bc_table);
+  if (md->uq_samples() > 0) {
+    std::vector<nsm::BcEntry> sample_table(table);
+    for (auto& e : sample_table)
+      for (int c = 0; c < 3; c++)
+        if (e.kind[c] == NSM_BC_PRESCRIBED_DISPLACEMENT) {
+          e.kind[c]  = 0;
+          e.value[c] = 0.0;
+        }
+    if ((st = dalloc(&md->uq_bc_table, std::max<size_t>(sample_table.size(), 1), &dummy))) return st;
+    if (!sample_table.empty())
+      NSM_HIP(hipMemcpy(md->uq_bc_table, sample_table.data(), sample_table.size() * sizeof(nsm::BcEntry),
+                        hipMemcpyHostToDevice));
+  }
   std::vector<int> tiles;
   const bool       have_rx = md->reaction_mask.size() == code.size();
   for (size_t i = 0; i < code.size(); i++)  // device node order
@@ -333,22 +376,24 @@ int check_status(nsm_model* md)
   return NSM_OK;
 }
 
-nsm::BlockTable<float> float_table(const nsm_model* md, int b0, int nb)
+nsm::BlockTable<float> float_table(const nsm_model* md, int b0, int nb, int sample)
 {
   nsm::BlockTable<float> bt{};
   bt.count = nb;
   for (int b = 0; b <= nb; b++) bt.off[b] = md->blk_off[b0 + b];
   for (int b = 0; b < nb; b++) {
-    bt.K[b] = (float)md->K[b0 + b];
-    bt.G[b] = (float)md->G[b0 + b];
+    bt.K[b] = (float)md->bulk(sample, b0 + b);
+    bt.G[b] = (float)md->shear(sample, b0 + b);
   }
   return bt;
 }
 
 // All blocks in one launch when they fit one block table, else groups of
-// kMaxLaunchBlocks consecutive blocks.
-int launch_elements(nsm_model* md)
+// kMaxLaunchBlocks consecutive blocks; displacement and moduli of the nominal
+// trajectory (sample < 0) or of exact UQ sample `sample`.
+int launch_elements(nsm_model* md, int sample)
 {
+  const double* u = md->node_arrays(sample).u;
   for (int b0 = 0; b0 < md->num_blocks; b0 += nsm::kMaxLaunchBlocks) {
     const int nb = std::min(nsm::kMaxLaunchBlocks, md->num_blocks - b0);
     size_t    ti;
@@ -359,14 +404,14 @@ int launch_elements(nsm_model* md)
       bt.count = nb;
       for (int b = 0; b <= nb; b++) bt.off[b] = md->blk_off[b0 + b];
       for (int b = 0; b < nb; b++) {
-        bt.K[b] = md->K[b0 + b];
-        bt.G[b] = md->G[b0 + b];
+        bt.K[b] = md->bulk(sample, b0 + b);
+        bt.G[b] = md->shear(sample, b0 + b);
       }
-      nsm::launch_element_fp64(bt, md->E, md->conn, md->X, md->u, (double*)md->Fip, (double*)md->sip,
+      nsm::launch_element_fp64(bt, md->E, md->conn, md->X, u, (double*)md->Fip, (double*)md->sip,
                                (double*)md->slots, md->status, md->stream);
     } else {
-      nsm::launch_element_mixed(float_table(md, b0, nb), md->E, md->conn, md->X, md->u, (float*)md->Fip, (float*)md->sip,
-                                (float*)md->slots, md->status, md->stream);
+      nsm::launch_element_mixed(float_table(md, b0, nb, sample), md->E, md->conn, md->X, u, (float*)md->Fip,
+                                (float*)md->sip, (float*)md->slots, md->status, md->stream);
     }
     NSM_LAUNCHED();
     if ((st = time_end(md, ti))) return st;
@@ -408,15 +453,15 @@ int launch_reaction(nsm_model* md, int64_t step)
 }
 
 // Node kernel of the slot path; the reaction partials of `step` are folded
-// into it.
-int launch_node(nsm_model* md, const nsm::StepArgs& sa, int64_t step)
+// into it (nominal trajectory only).
+int launch_node(nsm_model* md, const nsm::StepArgs& sa, int64_t step, int sample)
 {
   size_t ti;
   int    st = time_begin(md, NSM_KERNEL_NODE, &ti);
   if (st) return st;
   nsm::ReactionArgs     rx;
-  const bool            want_rx = reaction_args(md, step, &rx);
-  const nsm::NodeArrays na      = md->node_arrays();
+  const bool            want_rx = reaction_args(md, sample < 0 ? step : -1, &rx);
+  const nsm::NodeArrays na      = md->node_arrays(sample);
   if (md->precision == NSM_FP64)
     nsm::launch_node_update<double>(na, (const double*)md->slots, sa, rx, md->stream);
   else
@@ -430,11 +475,12 @@ int launch_node(nsm_model* md, const nsm::StepArgs& sa, int64_t step)
 // boundary-node update.  The reaction total of `step` is left to the caller
 // (`*pending` = step when it has one); the total of `fold_step` (a previous
 // step, or -1) is summed by the first element launch.
-int launch_tiled(nsm_model* md, const nsm::StepArgs& sa, int64_t step, int64_t fold_step, int64_t* pending)
+int launch_tiled(nsm_model* md, const nsm::StepArgs& sa, int64_t step, int64_t fold_step, int64_t* pending,
+                 int sample)
 {
   nsm::ReactionArgs     rx;
   const bool            want_rx = reaction_args(md, step, &rx);
-  const nsm::NodeArrays na      = md->node_arrays();
+  const nsm::NodeArrays na      = md->node_arrays(sample);
   nsm::ReactionFold     fold{nullptr, nullptr, 0, nullptr};
   if (fold_step >= 0)
     fold = nsm::ReactionFold{reaction_partials_of(md, fold_step), md->reaction_tiles, md->num_reaction_tiles,
@@ -446,7 +492,7 @@ int launch_tiled(nsm_model* md, const nsm::StepArgs& sa, int64_t step, int64_t f
     if (t1 <= t0) continue;  // no launch: the fold goes to the next one
     size_t ti;
     if ((st = time_begin(md, NSM_KERNEL_ELEMENT, &ti))) return st;
-    nsm::launch_element_tiled(float_table(md, b0, nb), md->E, md->tile_args, t0, t1 - t0, md->X, na,
+    nsm::launch_element_tiled(float_table(md, b0, nb, sample), md->E, md->tile_args, t0, t1 - t0, md->X, na,
                               (float*)md->Fip, (float*)md->sip, sa, rx, fold, md->status, md->stream);
     NSM_LAUNCHED();
     if ((st = time_end(md, ti))) return st;
@@ -461,15 +507,46 @@ int launch_tiled(nsm_model* md, const nsm::StepArgs& sa, int64_t step, int64_t f
   return NSM_OK;
 }
 
-// One step's force evaluation and node update (update_v = 0: f_int only).
-// The slot path finalizes the reaction of `step` itself; the tiled path
-// leaves it pending (see launch_tiled) and folds `fold_step` instead.
-int launch_step(nsm_model* md, const nsm::StepArgs& sa, int64_t step, int64_t fold_step, int64_t* pending)
+// One trajectory's force evaluation and node update (update_v = 0: f_int
+// only).  The slot path finalizes the reaction of `step` itself; the tiled
+// path leaves it pending (see launch_tiled) and folds `fold_step` instead.
+// UQ samples (sample >= 0) pass step = fold_step = -1: no reaction.
+int launch_trajectory(nsm_model* md, const nsm::StepArgs& sa, int64_t step, int64_t fold_step, int64_t* pending,
+                      int sample)
 {
   *pending = -1;
-  if (md->tiled) return launch_tiled(md, sa, step, fold_step, pending);
-  int st = launch_elements(md);
-  return st ? st : launch_node(md, sa, step);
+  if (md->tiled) return launch_tiled(md, sa, step, fold_step, pending, sample);
+  int st = launch_elements(md, sample);
+  return st ? st : launch_node(md, sa, step, sample);
+}
+
+// Closure forces of the approximate UQ samples from this step's nominal and
+// exact forces, with their node update.
+int launch_closure(nsm_model* md, const nsm::StepArgs& sa)
+{
+  size_t ti;
+  int    st = time_begin(md, NSM_KERNEL_CLOSURE, &ti);
+  if (st) return st;
+  const nsm::UqClosure uc{md->uq_exact, md->uq_approx, 3 * md->N, md->f, md->uq_state, md->uq_coeff};
+  nsm::launch_uq_closure(md->node_arrays(0), uc, sa, md->stream);
+  NSM_LAUNCHED();
+  return time_end(md, ti);
+}
+
+// One step of every trajectory.  Exact UQ samples run first: their element
+// launches write F and sigma into the nominal arrays (the reference keeps no
+// element data for them, src/uq/nimble_uq_block.cc:101-163), which the
+// nominal launch then overwrites.  Samples are independent of each other and
+// of the nominal trajectory until the closure, which reads this step's
+// nominal and exact forces.
+int launch_step(nsm_model* md, const nsm::StepArgs& sa, int64_t step, int64_t fold_step, int64_t* pending)
+{
+  int     st     = NSM_OK;
+  int64_t unused = -1;
+  for (int k = 0; k < md->uq_exact; k++)
+    if ((st = launch_trajectory(md, sa, -1, -1, &unused, k))) return st;
+  if ((st = launch_trajectory(md, sa, step, fold_step, pending, -1))) return st;
+  return md->uq_approx > 0 ? launch_closure(md, sa) : NSM_OK;
 }
 
 // Device copies of the tile arrays.
@@ -540,6 +617,40 @@ bool is_output(int64_t step, int64_t num_load_steps, int64_t output_frequency)
 {
   if (output_frequency == 0) return false;
   return step % output_frequency == 0 || step == num_load_steps - 1;
+}
+
+// device node array with w values per node -> host array in mesh node order
+int node_field_out(nsm_model* md, const double* src, size_t w, double* out)
+{
+  const size_t cnt = w * (size_t)md->N;
+  if (!md->tiled) {
+    NSM_HIP(hipMemcpyAsync(out, src, cnt * sizeof(double), hipMemcpyDeviceToHost, md->stream));
+    NSM_HIP(hipStreamSynchronize(md->stream));
+    return NSM_OK;
+  }
+  std::vector<double> tmp(cnt);
+  NSM_HIP(hipMemcpyAsync(tmp.data(), src, cnt * sizeof(double), hipMemcpyDeviceToHost, md->stream));
+  NSM_HIP(hipStreamSynchronize(md->stream));
+  for (int64_t n = 0; n < md->N; n++)
+    for (size_t c = 0; c < w; c++) out[(size_t)n * w + c] = tmp[(size_t)dev_node(md, n) * w + c];
+  return NSM_OK;
+}
+
+// host array in mesh node order -> device node array
+int node_field_in(nsm_model* md, double* dst, size_t w, const double* in)
+{
+  const size_t cnt = w * (size_t)md->N;
+  if (!md->tiled) {
+    NSM_HIP(hipMemcpyAsync(dst, in, cnt * sizeof(double), hipMemcpyHostToDevice, md->stream));
+    NSM_HIP(hipStreamSynchronize(md->stream));
+    return NSM_OK;
+  }
+  std::vector<double> tmp(cnt);
+  for (int64_t n = 0; n < md->N; n++)
+    for (size_t c = 0; c < w; c++) tmp[(size_t)dev_node(md, n) * w + c] = in[(size_t)n * w + c];
+  NSM_HIP(hipMemcpyAsync(dst, tmp.data(), cnt * sizeof(double), hipMemcpyHostToDevice, md->stream));
+  NSM_HIP(hipStreamSynchronize(md->stream));
+  return NSM_OK;
 }
 
 }  // namespace
@@ -810,6 +921,9 @@ int nsm_model_destroy(nsm_model* md)
   free_dev(md->reaction_hist);
   free_dev(md->reaction_partials);
   free_dev(md->reaction_tiles);
+  free_dev(md->uq_state);
+  free_dev(md->uq_coeff);
+  free_dev(md->uq_bc_table);
   for (void* p : md->tile_bufs) (void)hipFree(p);
   for (auto& p : md->event_pool) {
     (void)hipEventDestroy(p.start);
@@ -845,14 +959,19 @@ int nsm_model_apply_initial_conditions(nsm_model* md)
   if (!md) return fail(NSM_ERR_ARG, "null model");
   int st = ensure_device(md);
   if (st) return st;
+  // the nominal velocity and every UQ sample's
+  // (src/nimble_boundary_condition_manager.h:93-134)
   std::vector<double> v((size_t)(3 * md->N));
-  NSM_HIP(hipMemcpyAsync(v.data(), md->v, v.size() * sizeof(double), hipMemcpyDeviceToHost, md->stream));
-  NSM_HIP(hipStreamSynchronize(md->stream));
-  for (const auto& b : md->bcs)
-    if (b.kind == NSM_BC_INITIAL_VELOCITY)
-      for (int n : b.nodes) v[(size_t)(3 * dev_node(md, n) + b.coord)] = b.value;
-  NSM_HIP(hipMemcpyAsync(md->v, v.data(), v.size() * sizeof(double), hipMemcpyHostToDevice, md->stream));
-  NSM_HIP(hipStreamSynchronize(md->stream));
+  for (int s = -1; s < md->uq_samples(); s++) {
+    double* dv = md->node_arrays(s).v;
+    NSM_HIP(hipMemcpyAsync(v.data(), dv, v.size() * sizeof(double), hipMemcpyDeviceToHost, md->stream));
+    NSM_HIP(hipStreamSynchronize(md->stream));
+    for (const auto& b : md->bcs)
+      if (b.kind == NSM_BC_INITIAL_VELOCITY)
+        for (int n : b.nodes) v[(size_t)(3 * dev_node(md, n) + b.coord)] = b.value;
+    NSM_HIP(hipMemcpyAsync(dv, v.data(), v.size() * sizeof(double), hipMemcpyHostToDevice, md->stream));
+    NSM_HIP(hipStreamSynchronize(md->stream));
+  }
   return NSM_OK;
 }
 
@@ -862,8 +981,10 @@ int nsm_model_apply_kinematic_conditions(nsm_model* md, double time_current, dou
   int st = ensure_device(md);
   if (st || (st = rebuild_bcs(md))) return st;
   nsm::StepTimes t{time_previous, time_current, time_current - time_previous, 0.0};
-  nsm::launch_apply_bc(md->node_arrays(), t, md->stream);
-  NSM_LAUNCHED();
+  for (int s = -1; s < md->uq_samples(); s++) {
+    nsm::launch_apply_bc(md->node_arrays(s), t, md->stream);
+    NSM_LAUNCHED();
+  }
   return NSM_OK;
 }
 
@@ -1023,8 +1144,10 @@ int nsm_model_explicit_steps(nsm_model* md, int64_t step_begin, int64_t step_end
 
   double         t   = *time_current;
   nsm::StepTimes cur = step_times(&t, dt_user);
-  nsm::launch_prekick(md->node_arrays(), cur, md->stream);
-  NSM_LAUNCHED();
+  for (int s = -1; s < md->uq_samples(); s++) {
+    nsm::launch_prekick(md->node_arrays(s), cur, md->stream);
+    NSM_LAUNCHED();
+  }
   int64_t pending = -1;  // step whose reaction total is not yet summed
   for (int64_t step = step_begin; step < step_end; step++) {
     const bool     last = step + 1 == step_end;
@@ -1086,19 +1209,7 @@ int nsm_model_get_field(nsm_model* md, int field, double* out)
     }
     default: return fail(NSM_ERR_ARG, "unknown field %d", field);
   }
-  if (!md->tiled) {
-    NSM_HIP(hipMemcpyAsync(out, src, cnt * sizeof(double), hipMemcpyDeviceToHost, md->stream));
-    NSM_HIP(hipStreamSynchronize(md->stream));
-    return NSM_OK;
-  }
-  // device node order -> mesh node order
-  std::vector<double> tmp(cnt);
-  NSM_HIP(hipMemcpyAsync(tmp.data(), src, cnt * sizeof(double), hipMemcpyDeviceToHost, md->stream));
-  NSM_HIP(hipStreamSynchronize(md->stream));
-  const size_t w = cnt / (size_t)md->N;
-  for (int64_t n = 0; n < md->N; n++)
-    for (size_t c = 0; c < w; c++) out[(size_t)n * w + c] = tmp[(size_t)dev_node(md, n) * w + c];
-  return NSM_OK;
+  return node_field_out(md, src, cnt / (size_t)md->N, out);
 }
 
 int nsm_model_set_field(nsm_model* md, int field, const double* in)
@@ -1121,19 +1232,84 @@ int nsm_model_set_field(nsm_model* md, int field, const double* in)
       break;
     default: return fail(NSM_ERR_ARG, "field %d cannot be set", field);
   }
-  if (!md->tiled) {
-    NSM_HIP(hipMemcpyAsync(dst, in, cnt * sizeof(double), hipMemcpyHostToDevice, md->stream));
-    NSM_HIP(hipStreamSynchronize(md->stream));
-    return NSM_OK;
-  }
-  // mesh node order -> device node order
-  std::vector<double> tmp(cnt);
-  const size_t        w = cnt / (size_t)md->N;
-  for (int64_t n = 0; n < md->N; n++)
-    for (size_t c = 0; c < w; c++) tmp[(size_t)dev_node(md, n) * w + c] = in[(size_t)n * w + c];
-  NSM_HIP(hipMemcpyAsync(dst, tmp.data(), cnt * sizeof(double), hipMemcpyHostToDevice, md->stream));
+  return node_field_in(md, dst, cnt / (size_t)md->N, in);
+}
+
+int nsm_model_set_uq_samples(nsm_model* md, int num_exact, const double* bulk_modulus, const double* shear_modulus,
+                             int num_approx, const double* coefficients)
+{
+  if (!md) return fail(NSM_ERR_ARG, "null model");
+  if (num_exact < 0 || num_approx < 0) return fail(NSM_ERR_ARG, "negative sample count");
+  if (num_exact > 0 && (!bulk_modulus || !shear_modulus)) return fail(NSM_ERR_ARG, "null sample moduli");
+  if (num_approx > 0 && !coefficients) return fail(NSM_ERR_ARG, "null closure coefficients");
+  const size_t nk = (size_t)num_exact * (size_t)md->num_blocks;
+  const size_t nc = (size_t)num_approx * (size_t)(num_exact + 1);
+  for (size_t i = 0; i < nk; i++)
+    if (!std::isfinite(bulk_modulus[i]) || !std::isfinite(shear_modulus[i]))
+      return fail(NSM_ERR_ARG, "non-finite modulus for exact sample %zu", i / (size_t)md->num_blocks);
+  for (size_t i = 0; i < nc; i++)
+    if (!std::isfinite(coefficients[i]))
+      return fail(NSM_ERR_ARG, "non-finite closure coefficient for sample %zu", i / (size_t)(num_exact + 1));
+  int st = ensure_device(md);
+  if (st) return st;
   NSM_HIP(hipStreamSynchronize(md->stream));
+  const size_t state_doubles = 12 * (size_t)md->N * (size_t)(num_exact + num_approx);
+  const size_t old_bytes     = (12 * (size_t)md->N * (size_t)md->uq_samples() +
+                            (size_t)md->uq_approx * (size_t)(md->uq_exact + 1)) * sizeof(double);
+  free_dev(md->uq_state);
+  free_dev(md->uq_coeff);
+  free_dev(md->uq_bc_table);
+  md->device_bytes -= std::min(md->device_bytes, old_bytes);
+  md->uq_exact = md->uq_approx = 0;
+  md->uq_K.clear();
+  md->uq_G.clear();
+  md->bc_dirty = true;
+  if (num_exact + num_approx == 0) return NSM_OK;
+  if ((st = dalloc(&md->uq_state, state_doubles, &md->device_bytes)) ||
+      (st = dalloc(&md->uq_coeff, nc, &md->device_bytes)))
+    return st;
+  NSM_HIP(hipMemsetAsync(md->uq_state, 0, state_doubles * sizeof(double), md->stream));
+  if (nc) NSM_HIP(hipMemcpyAsync(md->uq_coeff, coefficients, nc * sizeof(double), hipMemcpyHostToDevice, md->stream));
+  NSM_HIP(hipStreamSynchronize(md->stream));
+  md->uq_K.assign(bulk_modulus, bulk_modulus + nk);
+  md->uq_G.assign(shear_modulus, shear_modulus + nk);
+  md->uq_exact  = num_exact;
+  md->uq_approx = num_approx;
   return NSM_OK;
+}
+
+// node array of field `field` of UQ sample `sample`
+static int uq_field(nsm_model* md, int sample, int field, double** p)
+{
+  if (sample < 0 || sample >= md->uq_samples())
+    return fail(NSM_ERR_ARG, "UQ sample %d out of range (%d samples)", sample, md->uq_samples());
+  const nsm::NodeArrays na = md->node_arrays(sample);
+  switch (field) {
+    case NSM_FIELD_DISPLACEMENT: *p = na.u; break;
+    case NSM_FIELD_VELOCITY: *p = na.v; break;
+    case NSM_FIELD_ACCELERATION: *p = na.a; break;
+    case NSM_FIELD_INTERNAL_FORCE: *p = na.f; break;
+    default: return fail(NSM_ERR_ARG, "field %d is not a UQ sample field", field);
+  }
+  return NSM_OK;
+}
+
+int nsm_model_get_uq_field(nsm_model* md, int sample, int field, double* out)
+{
+  if (!md || !out) return fail(NSM_ERR_ARG, "null argument");
+  double* p  = nullptr;
+  int     st = ensure_device(md);
+  if (st || (st = uq_field(md, sample, field, &p))) return st;
+  return node_field_out(md, p, 3, out);
+}
+
+int nsm_model_set_uq_field(nsm_model* md, int sample, int field, const double* in)
+{
+  if (!md || !in) return fail(NSM_ERR_ARG, "null argument");
+  double* p  = nullptr;
+  int     st = ensure_device(md);
+  if (st || (st = uq_field(md, sample, field, &p))) return st;
+  return node_field_in(md, p, 3, in);
 }
 
 int nsm_model_field_device_pointer(nsm_model* md, int field, void** ptr, int* elem_bytes)

@@ -148,6 +148,55 @@ __global__ __launch_bounds__(kNodeThreads) void k_sample_history(const double* _
   out[k]           = (1.0 - w) * r0 + w * r1;
 }
 
+// UQ closure (UqModel::ApplyClosure, src/uq/nimble_uq.cc:327-346) fused with
+// the approximate samples' node update: per node, f_s = f_nominal * c0, then
+// f_s += f_exact_k * c_{k+1} for k ascending, then node_step with the
+// sample's u, v, a, f.  The first kCachedExact exact forces of the node stay
+// in registers across the samples.
+constexpr int kCachedExact = 8;
+
+__global__ __launch_bounds__(kNodeThreads) void k_uq_closure(NodeArrays na, UqClosure uc, StepArgs sa)
+{
+  const int64_t i = (int64_t)xcd_tile(blockIdx.x, gridDim.x) * kNodeThreads + threadIdx.x;
+  if (i >= na.num_nodes) return;
+  const int ne = uc.num_exact;
+  double    fn[3], fe[kCachedExact][3];
+  load3(uc.f_nominal, i, fn);
+#pragma unroll
+  for (int k = 0; k < kCachedExact; k++)
+    if (k < ne) load3(uc.state + (4 * (int64_t)k + 3) * uc.stride, i, fe[k]);
+  const int flags = na.flags[i];
+  for (int s = 0; s < uc.num_approx; s++) {
+    const double* c  = uc.coeff + (int64_t)s * (ne + 1);
+    double        f[3] = {fn[0] * c[0], fn[1] * c[0], fn[2] * c[0]};
+#pragma unroll
+    for (int k = 0; k < kCachedExact; k++)
+      if (k < ne) {
+        const double ck = c[k + 1];
+        f[0] += fe[k][0] * ck;
+        f[1] += fe[k][1] * ck;
+        f[2] += fe[k][2] * ck;
+      }
+    for (int k = kCachedExact; k < ne; k++) {
+      double g[3];
+      load3(uc.state + (4 * (int64_t)k + 3) * uc.stride, i, g);
+      const double ck = c[k + 1];
+      f[0] += g[0] * ck;
+      f[1] += g[1] * ck;
+      f[2] += g[2] * ck;
+    }
+    NodeArrays ns = na;
+    double*    b  = uc.state + 4 * (int64_t)(ne + s) * uc.stride;
+    ns.u          = b;
+    ns.v          = b + uc.stride;
+    ns.a          = b + 2 * uc.stride;
+    ns.f          = b + 3 * uc.stride;
+    double u[3];
+    if (sa.update_v) load3(ns.u, i, u);
+    node_step(ns, i, flags, f, u, sa);
+  }
+}
+
 }  // namespace
 
 void launch_sample_history(const double* hist, int64_t n, int num_samples, double* out, hipStream_t s)
@@ -182,6 +231,12 @@ void launch_node_update(const NodeArrays& na, const SlotT* slots, const StepArgs
 template void launch_node_update<double>(const NodeArrays&, const double*, const StepArgs&, ReactionArgs,
                                          hipStream_t);
 template void launch_node_update<float>(const NodeArrays&, const float*, const StepArgs&, ReactionArgs, hipStream_t);
+
+void launch_uq_closure(const NodeArrays& na, const UqClosure& uc, const StepArgs& sa, hipStream_t s)
+{
+  if (na.num_nodes <= 0 || uc.num_approx <= 0) return;
+  hipLaunchKernelGGL(k_uq_closure, dim3(node_grid(na.num_nodes)), dim3(kNodeThreads), 0, s, na, uc, sa);
+}
 
 void launch_reaction_finalize(const double* partials, const int* tiles, int num_tiles, double* out, hipStream_t s)
 {

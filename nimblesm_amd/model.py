@@ -130,6 +130,7 @@ class ModelData:
         self.num_nodes = X.shape[0]
         self.num_elem = conn.shape[0]
         self.block_offsets = np.concatenate([[0], np.cumsum(ne)])
+        self.num_exact_samples = self.num_approx_samples = 0
 
     # -- lifetime
     def close(self):
@@ -251,6 +252,32 @@ class ModelData:
         out = np.zeros(num_samples)
         check(lib().nsm_model_sample_reaction_history(self._h, int(num_steps), int(num_samples), out.ctypes.data))
         return out
+
+    # -- off-nominal ensemble (src/uq/nimble_uq_model_data.cc:145-248)
+    def SetUqSamples(self, bulk_modulus, shear_modulus, coefficients):
+        """Exact samples with moduli bulk_modulus, shear_modulus [num_exact,
+        num_blocks] (blocks in ascending block id) and approximate samples
+        with closure coefficients [num_approx, num_exact + 1]; every sample
+        starts at rest.  Empty arrays remove the ensemble."""
+        nb = len(self.blocks)
+        K = np.ascontiguousarray(bulk_modulus, dtype=np.float64).reshape(-1, nb)
+        G = np.ascontiguousarray(shear_modulus, dtype=np.float64).reshape(-1, nb)
+        if K.shape != G.shape:
+            raise ValueError("bulk and shear moduli need the same shape")
+        ne = K.shape[0]
+        c = np.ascontiguousarray(coefficients, dtype=np.float64).reshape(-1, ne + 1)
+        check(lib().nsm_model_set_uq_samples(self._h, ne, K.ctypes.data, G.ctypes.data, c.shape[0], c.ctypes.data))
+        self.num_exact_samples, self.num_approx_samples = ne, c.shape[0]
+
+    def GetUqNodeData(self, sample: int, label: str) -> np.ndarray:
+        """Node vector `label` of UQ sample `sample` (exact samples first)."""
+        out = np.zeros((self.num_nodes, 3))
+        check(lib().nsm_model_get_uq_field(self._h, int(sample), FIELDS[label], out.ctypes.data))
+        return out
+
+    def SetUqNodeData(self, sample: int, label: str, values: np.ndarray):
+        v = np.ascontiguousarray(values, dtype=np.float64).reshape(self.num_nodes, 3)
+        check(lib().nsm_model_set_uq_field(self._h, int(sample), FIELDS[label], v.ctypes.data))
 
     # -- instrumentation
     def Synchronize(self):

@@ -347,3 +347,110 @@ def sample_history(R, num_samples=64):
         r1 = full[j + 1] if j < n else r0
         out[k] = (1.0 - w) * r0 + w * r1
     return out
+
+
+def _sample_bcs(model: Model) -> Model:
+    """The model's mesh with only the conditions UQ samples take: initial and
+    prescribed velocities (prescribed displacements set the nominal velocity
+    only, src/nimble_boundary_condition_manager.h:184-215)."""
+    keep = model.bc_kind != 2
+    return Model(model.X, model.conn, model.blk_off, model.rho, model.K, model.G, model.bc_node[keep],
+                 model.bc_coord[keep], model.bc_kind[keep], model.bc_value[keep])
+
+
+class UqRun:
+    """The reference's off-nominal ensemble (UQ) on the oracle.
+
+    * nominal trajectory: ExplicitRun of the model;
+    * exact sample k: ExplicitRun of the model with block moduli
+      K_exact[k], G_exact[k] (Block::ComputeInternalForce with
+      use_alternative_parameters, src/uq/nimble_uq_block.cc:57-164) and the
+      samples' conditions; its update is UpdateWithNewVelocity /
+      UpdateWithNewDisplacement (src/uq/nimble_uq_model_data.cc:222-248),
+      the same arithmetic as the nominal one;
+    * approximate sample s: force f_s = f_nom * c[s,0], then
+      f_s += f_exact_k * c[s,k+1] for k ascending (UqModel::ApplyClosure,
+      src/uq/nimble_uq.cc:327-346) after each step's forces, and the same
+      half kick / drift / conditions as the other trajectories
+      (src/integrators/explicit_time_integrator.cc:197-270).
+
+    Trajectories are independent until the closure, so they are advanced one
+    after another within each step.  All share the lumped mass.
+    """
+
+    def __init__(self, model: Model, K_exact, G_exact, coeff, t_final: float, num_load_steps: int,
+                 output_frequency: int = 1, t_initial: float = 0.0, reaction_nodes=None, reaction_coord: int = 2):
+        nb = len(model.rho)
+        K_exact = np.asarray(K_exact, np.float64).reshape(-1, nb)
+        G_exact = np.asarray(G_exact, np.float64).reshape(-1, nb)
+        self.coeff = np.asarray(coeff, np.float64).reshape(-1, K_exact.shape[0] + 1)
+        kw = dict(output_frequency=output_frequency, t_initial=t_initial)
+        self.nominal = ExplicitRun(model, t_final, num_load_steps, reaction_nodes=reaction_nodes,
+                                   reaction_coord=reaction_coord, **kw)
+        self.samples_model = sm = _sample_bcs(model)
+        self.exact = []
+        for k in range(K_exact.shape[0]):
+            mk = Model(sm.X, sm.conn, sm.blk_off, sm.rho, K_exact[k], G_exact[k], sm.bc_node, sm.bc_coord,
+                       sm.bc_kind, sm.bc_value)
+            self.exact.append(ExplicitRun(mk, t_final, num_load_steps, **kw))
+        na, N = self.coeff.shape[0], model.num_nodes
+        self.u = np.zeros((na, N, 3))
+        self.v = np.zeros((na, N, 3))
+        self.f = np.zeros((na, N, 3))
+        self.m = self.nominal.m
+        self.num_load_steps = num_load_steps
+        self.output_frequency = output_frequency
+        self.dt_user = self.nominal.dt_user
+        self.time_current = t_initial
+        self.step = 0
+        for s in range(na):
+            lib().orc_apply_initial_conditions(C.byref(sm._bcs), self.v[s].reshape(-1))
+            lib().orc_apply_kinematic_bc(C.byref(sm._bcs), 0.0, 0.0, self.u[s].reshape(-1), self.v[s].reshape(-1))
+
+    @property
+    def num_samples(self):
+        return len(self.exact) + self.coeff.shape[0]
+
+    def sample(self, i: int):
+        """(u, v, f) of sample i, exact samples first."""
+        if i < len(self.exact):
+            r = self.exact[i]
+            return r.u, r.v, r.f
+        s = i - len(self.exact)
+        return self.u[s], self.v[s], self.f[s]
+
+    def _bc(self, t_cur, t_prev, s):
+        lib().orc_apply_kinematic_bc(C.byref(self.samples_model._bcs), t_cur, t_prev, self.u[s].reshape(-1),
+                                     self.v[s].reshape(-1))
+
+    def advance(self, nsteps: int):
+        end = min(self.step + nsteps, self.num_load_steps)
+        inv_m = (1.0 / self.m)[:, None]
+        for step in range(self.step, end):
+            out = self.output_frequency != 0 and (step % self.output_frequency == 0
+                                                  or step == self.num_load_steps - 1)
+            t_prev = self.time_current
+            t_cur = t_prev + self.dt_user
+            delta = t_cur - t_prev
+            half = 0.5 * delta
+            self.nominal.advance(1)
+            for r in self.exact:
+                r.advance(1)
+            for s in range(self.coeff.shape[0]):
+                u, v = self.u[s], self.v[s]
+                # first half: a = (1/m)(f + f_ext), v += dt/2 a, BC, u += dt v, BC
+                v += (half * 1.0) * (inv_m * (self.f[s] + 0.0))
+                self._bc(t_cur, t_prev, s)
+                u += (delta * 1.0) * v
+                self._bc(t_cur, t_prev, s)
+                c = self.coeff[s]
+                f = self.nominal.f * c[0]
+                for k, r in enumerate(self.exact):
+                    f = f + r.f * c[k + 1]
+                self.f[s] = f
+                v += (half * 1.0) * (inv_m * (f + 0.0))
+                if out:
+                    self._bc(t_cur, t_prev, s)
+            self.time_current = t_cur
+        self.step = end
+        return self

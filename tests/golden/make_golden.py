@@ -1,8 +1,9 @@
 """Extract golden fixtures from NimbleSM's own regression data.
 
 Reads (as data only) the genesis meshes, input decks and ``*.gold.e`` Exodus
-results under ``/root/reference/test/dynamics`` with ``scipy.io.netcdf_file``
-and writes one compressed ``.npz`` per deck next to this script.  Nothing of
+results under ``/root/reference/test/dynamics`` and ``/root/reference/test/uq``
+with ``scipy.io.netcdf_file`` and writes one compressed ``.npz`` per deck next
+to this script.  Nothing of
 the reference is imported or executed.  Run in a container that has the
 reference tree:
 
@@ -14,6 +15,10 @@ Fixture contents (all 0-based):
   t_final, num_load_steps, output_frequency,
   gold_time [T], gold_<nodal var> [T,N], gold_<elem var> [T,E] (element order
   as in conn).
+UQ decks (uq_wave; its gold file is not in the reference checkout, so no
+gold_* arrays) add: uq_model (the 'uq model' value), uq_material / uq_line
+(the 'uq parameters' lines: material key, rest of the line), uq_samples_text
+(the samples file named by uq_model, as text), output_fields.
 """
 from __future__ import annotations
 
@@ -24,16 +29,18 @@ import sys
 import numpy as np
 from scipy.io import netcdf_file
 
-REF = "/root/reference/test/dynamics"
+REF = "/root/reference/test"
 OUT = os.path.dirname(os.path.abspath(__file__))
 
 DECKS = {
-    "single_elem_native_neohookean": ("single_elem_complex_displacement", "single_elem.g"),
-    "simple_deformation_modes": ("simple_deformation_modes", "simple_deformation_modes.g"),
-    "rigid_body_motion": ("rigid_body_motion", "rigid_body_motion.g"),
-    "wave_in_bar": ("wave_in_bar", "wave_in_bar.g"),
-    "brick_with_fibers": ("brick_with_fibers", "brick_with_fibers.g"),
-    "notched_plate_native_neohookean": ("notched_plate_native_neohookean", "notched_plate_native_neohookean.g"),
+    "single_elem_native_neohookean": ("dynamics/single_elem_complex_displacement", "single_elem.g"),
+    "simple_deformation_modes": ("dynamics/simple_deformation_modes", "simple_deformation_modes.g"),
+    "rigid_body_motion": ("dynamics/rigid_body_motion", "rigid_body_motion.g"),
+    "wave_in_bar": ("dynamics/wave_in_bar", "wave_in_bar.g"),
+    "brick_with_fibers": ("dynamics/brick_with_fibers", "brick_with_fibers.g"),
+    "notched_plate_native_neohookean": ("dynamics/notched_plate_native_neohookean",
+                                        "notched_plate_native_neohookean.g"),
+    "uq_wave": ("uq/uq_wave", "uq_wave.g"),
 }
 
 COORD = {"x": 0, "y": 1, "z": 2}
@@ -62,7 +69,7 @@ def read_genesis(path):
 
 
 def parse_deck(path):
-    deck = {"materials": {}, "blocks": [], "bcs": []}
+    deck = {"materials": {}, "blocks": [], "bcs": [], "uq": {}}
     for line in open(path):
         line = line.split("#")[0].strip()
         if ":" not in line:
@@ -88,6 +95,13 @@ def parse_deck(path):
         elif key == "boundary condition":
             kind, ns, coord, mag = val.split()
             deck["bcs"].append((KIND[kind], int(re.sub(r"\D", "", ns)), COORD[coord], float(mag)))
+        elif key == "uq parameters":
+            mat, rest = val.split(None, 1)
+            deck["uq"][mat] = rest.strip()
+        elif key == "uq model":
+            deck["uq_model"] = val
+        elif key == "output fields":
+            deck["output_fields"] = val
     deck.setdefault("t_initial", 0.0)
     deck.setdefault("output_frequency", 1)
     return deck
@@ -97,7 +111,8 @@ def make(name, subdir, mesh):
     d = os.path.join(REF, subdir)
     X, blocks, ns = read_genesis(os.path.join(d, mesh))
     deck = parse_deck(os.path.join(d, name + ".in"))
-    gold = netcdf_file(os.path.join(d, name + ".gold.e"), "r", mmap=False)
+    gold_path = os.path.join(d, name + ".gold.e")
+    gold = netcdf_file(gold_path, "r", mmap=False) if os.path.exists(gold_path) else None
 
     mat_of_block = dict(deck["blocks"])
     order = sorted(blocks)  # ascending block id, as std::map in ModelData
@@ -125,8 +140,19 @@ def make(name, subdir, mesh):
         bc_node=np.array(bn, np.int32), bc_coord=np.array(bc, np.int32), bc_kind=np.array(bk, np.int32),
         bc_value=np.array(bv), t_final=deck["t_final"], t_initial=deck["t_initial"],
         num_load_steps=deck["num_load_steps"], output_frequency=deck["output_frequency"],
-        gold_time=gold.variables["time_whole"][:].astype(np.float64),
     )
+    if deck["uq"]:
+        mats = sorted(deck["uq"])
+        out.update(uq_model=deck["uq_model"], uq_material=np.array(mats), uq_line=np.array([deck["uq"][m] for m in mats]),
+                   output_fields=deck.get("output_fields", ""))
+        fname = deck["uq_model"].split()[1]
+        out["uq_samples_text"] = open(os.path.join(d, fname)).read()
+    if gold is None:
+        path = os.path.join(OUT, name + ".npz")
+        np.savez_compressed(path, **out)
+        print(f"{name}: N={X.shape[0]} E={conn.shape[0]} (no gold file) -> {path}")
+        return
+    out["gold_time"] = gold.variables["time_whole"][:].astype(np.float64)
     # gold results: nodal vars by name, element vars concatenated over blocks in ascending id
     gX = np.stack([gold.variables[c][:] for c in ("coordx", "coordy", "coordz")], axis=1)
     assert np.array_equal(gX, X), "gold mesh differs from genesis mesh"

@@ -127,3 +127,35 @@ def test_exodus_records(tmp_path):
         # ipt03_deformation_gradient_xy (K_F slot 3) of block 1 at record 0
         F0 = fake.GetIntegrationPointData("deformation_gradient") - 2.0
         assert np.array_equal(f.variables["vals_elem_var10eb1"][0, :], F0[:e0, 2, 3])
+
+
+class FakeUqModel(FakeModel):
+    num_exact_samples = 2
+
+    def GetUqNodeData(self, sample, label):
+        return np.full((self.N, 3), 1000.0 * self.k + 10.0 * (sample + 1) + (label == "internal_force"))
+
+
+def test_exodus_uq_sample_fields(tmp_path):
+    """exact_<q>_<k> is sample k-1, sample_<q>_<k> is sample num_exact + k-1,
+    one row per record"""
+    mesh = cube_mesh()
+    E = sum(len(c) for c in mesh.blocks.values())
+    path = str(tmp_path / "out.e")
+    fields = ["displacement", "exact_displacement_2", "sample_force_1", "sample_displacement_3"]
+    w = D.ExodusWriter(path, mesh, fields)
+    fake = FakeUqModel(mesh.num_nodes, E)
+    for k in range(4):
+        fake.k = k
+        w.write(1.0e-6 * k, fake)
+    w.close()
+    with pytest.raises(ValueError, match="not available"):
+        D.ExodusWriter(str(tmp_path / "bad.e"), mesh, ["sample_stress_1"])
+    with netcdf_file(path, "r", mmap=False) as f:
+        names = [D._chars_to_str(r) for r in f.variables["name_nod_var"][:]]
+        assert names == [f"{q}_{c}" for q in fields for c in "xyz"]
+        k = np.arange(4) * 1000.0
+        col = {n: f.variables[f"vals_nod_var{i + 1}"][:, 0] for i, n in enumerate(names)}
+        assert np.array_equal(col["exact_displacement_2_y"], k + 20.0)
+        assert np.array_equal(col["sample_force_1_z"], k + 31.0)
+        assert np.array_equal(col["sample_displacement_3_x"], k + 50.0)
