@@ -178,9 +178,12 @@ int nsm_model_get_field(nsm_model* model, int field, double* out);
 int nsm_model_set_field(nsm_model* model, int field, const double* in);
 /* Device address of a field (node fields double; ipt fields double in FP64
  * mode and float in mixed mode) and its element size in bytes.  Node fields
- * are [N][3] / [N] in the node order of nsm_model_get_node_order; ipt fields
- * are element-fastest [comp][ipt][E] in the element order of
- * nsm_model_get_element_order. */
+ * are [N][3] / [N] in the node order of nsm_model_get_node_order.  Ipt
+ * fields (nc = 9 or 6 components, device elements in the order of
+ * nsm_model_get_element_order) are element-fastest [comp][ipt][E] in FP64 and
+ * untiled mixed models; tiled mixed models store them per element tile: the
+ * tile holding device elements [e0, e0 + n) owns [8 nc e0, 8 nc (e0 + n)),
+ * value (c, q) of its element e0 + l at 8 nc e0 + (8 c + q) n + l. */
 int nsm_model_field_device_pointer(nsm_model* model, int field, void** ptr, int* elem_bytes);
 /* Node order of the device node arrays: out[i] (i < N) is the mesh node
  * stored at device position i.  FP64 models keep the mesh numbering; tiled

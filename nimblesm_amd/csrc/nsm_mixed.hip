@@ -49,11 +49,12 @@ __global__ __launch_bounds__(kThreads) NSM_MIXED_WAVES_ATTR void k_element_mixed
     ue[j][1]        = u[3 * n + 1];
     ue[j][2]        = u[3 * n + 2];
   }
+  // F and sigma are not read again by the step: non-temporal stores
   auto store = [&](int q, const float* Fq, const float* Sq) {
 #pragma unroll
-    for (int c = 0; c < 9; c++) Fip[(int64_t)(c * 8 + q) * E_total + e] = Fq[c];
+    for (int c = 0; c < 9; c++) __builtin_nontemporal_store(Fq[c], Fip + (int64_t)(c * 8 + q) * E_total + e);
 #pragma unroll
-    for (int c = 0; c < 6; c++) sip[(int64_t)(c * 8 + q) * E_total + e] = Sq[c];
+    for (int c = 0; c < 6; c++) __builtin_nontemporal_store(Sq[c], sip + (int64_t)(c * 8 + q) * E_total + e);
   };
   float      f[8][3];
   const bool ok = mixed::element(Xe, ue, K, G, store, f);

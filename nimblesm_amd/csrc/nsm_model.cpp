@@ -127,8 +127,8 @@ struct nsm_model {
   int*      conn = nullptr;
   uint32_t* node_ptr = nullptr;
   uint32_t* node_ent = nullptr;
-  void*     Fip = nullptr;       // double or float [9][8][E] (component, ipt, element)
-  void*     sip = nullptr;       // double or float [6][8][E]
+  void*     Fip = nullptr;       // double or float [9][8][E] (component, ipt, element); tiled: per tile
+  void*     sip = nullptr;       // double or float [6][8][E]; tiled: per tile (see ip_index)
   void*     slots = nullptr;     // double or float [3][8][E] element force slots
   double*   elem_tmp = nullptr;  // [8][E] setup scratch (element masses)
   double*   wg_min = nullptr;
@@ -223,6 +223,28 @@ inline int64_t element_of(const nsm_model* md, int64_t e)
 // device index of mesh node n, and mesh node at device index i
 inline int64_t dev_node(const nsm_model* md, int64_t n) { return md->tiled ? md->tiles.node_new[(size_t)n] : n; }
 inline int64_t mesh_node(const nsm_model* md, int64_t i) { return md->tiled ? md->tiles.node_old[(size_t)i] : i; }
+
+// Calls f(device index, device element, component, point) for every value of
+// an integration-point field with nc components: element-fastest rows
+// [c][q][E] in untiled models; in tiled ones each tile t (device elements
+// [e0, e0 + n)) owns [8 nc e0, 8 nc (e0 + n)) with rows (8 c + q) of length n.
+template <typename Fn>
+void for_each_ip(const nsm_model* md, int nc, Fn&& fn)
+{
+  if (!md->tiled) {
+    for (int c = 0; c < nc; c++)
+      for (int q = 0; q < 8; q++)
+        for (int64_t e = 0; e < md->E; e++) fn((int64_t)(c * 8 + q) * md->E + e, e, c, q);
+    return;
+  }
+  const auto& eo = md->tiles.elem_off;
+  for (int t = 0; t < md->tiles.num_tiles; t++) {
+    const int64_t e0 = eo[(size_t)t], n = eo[(size_t)t + 1] - e0, base = 8 * nc * e0;
+    for (int c = 0; c < nc; c++)
+      for (int q = 0; q < 8; q++)
+        for (int64_t l = 0; l < n; l++) fn(base + (c * 8 + q) * n + l, e0 + l, c, q);
+  }
+}
 
 int ensure_device(nsm_model* md)
 {
@@ -492,7 +514,7 @@ int launch_tiled(nsm_model* md, const nsm::StepArgs& sa, int64_t step, int64_t f
     if (t1 <= t0) continue;  // no launch: the fold goes to the next one
     size_t ti;
     if ((st = time_begin(md, NSM_KERNEL_ELEMENT, &ti))) return st;
-    nsm::launch_element_tiled(float_table(md, b0, nb, sample), md->E, md->tile_args, t0, t1 - t0, md->X, na,
+    nsm::launch_element_tiled(float_table(md, b0, nb, sample), md->tile_args, t0, t1 - t0, md->X, na,
                               (float*)md->Fip, (float*)md->sip, sa, rx, fold, md->status, md->stream);
     NSM_LAUNCHED();
     if ((st = time_end(md, ti))) return st;
@@ -878,15 +900,16 @@ int nsm_model_create(int64_t num_nodes, const double* reference_coordinates, int
   NSM_HIP_C(hipMemsetAsync(md->flags, 0, (size_t)num_nodes, s));
   if (E > 0) {
     // STEP_NP1 deformation gradient starts at the identity, stress at zero
-    // (src/nimble_kokkos_model_data.cc:921-924); [comp][ipt][E] layout, the
-    // diagonal components XX, YY, ZZ come first
+    // (src/nimble_kokkos_model_data.cc:921-924); the diagonal components XX,
+    // YY, ZZ are components 0-2
     std::vector<char> ident((size_t)(72 * E) * ipw, 0);
-    for (int64_t p = 0; p < 24 * E; p++) {
+    for_each_ip(md, 9, [&](int64_t p, int64_t, int c, int) {
+      if (c >= 3) return;
       if (precision == NSM_FP64)
         ((double*)ident.data())[p] = 1.0;
       else
         ((float*)ident.data())[p] = 1.0f;
-    }
+    });
     NSM_HIP_C(hipMemcpy(md->Fip, ident.data(), ident.size(), hipMemcpyHostToDevice));
     NSM_HIP_C(hipMemsetAsync(md->sip, 0, (size_t)(48 * E) * ipw, s));
   }
@@ -1184,7 +1207,7 @@ int nsm_model_get_field(nsm_model* md, int field, double* out)
     case NSM_FIELD_LUMPED_MASS: src = md->m, cnt = (size_t)md->N; break;
     case NSM_FIELD_DEFORMATION_GRADIENT:
     case NSM_FIELD_STRESS: {
-      // device [comp][ipt][E] -> host [E][ipt][comp] (the reference's element-data order)
+      // device layout (for_each_ip) -> host [E][ipt][comp] (the reference's element-data order)
       const int    nc  = field == NSM_FIELD_DEFORMATION_GRADIENT ? 9 : 6;
       const void*  p   = field == NSM_FIELD_DEFORMATION_GRADIENT ? md->Fip : md->sip;
       const size_t E   = (size_t)md->E;
@@ -1200,11 +1223,9 @@ int nsm_model_get_field(nsm_model* md, int field, double* out)
         NSM_HIP(hipStreamSynchronize(md->stream));
         for (size_t i = 0; i < cnt; i++) tmp[i] = tf[i];
       }
-      for (size_t c = 0; c < (size_t)nc; c++)
-        for (size_t q = 0; q < 8; q++) {
-          const double* src_row = tmp.data() + (c * 8 + q) * E;
-          for (size_t e = 0; e < E; e++) out[((size_t)element_of(md, (int64_t)e) * 8 + q) * nc + c] = src_row[e];
-        }
+      for_each_ip(md, nc, [&](int64_t i, int64_t e, int c, int q) {
+        out[((size_t)element_of(md, e) * 8 + (size_t)q) * (size_t)nc + (size_t)c] = tmp[(size_t)i];
+      });
       return NSM_OK;
     }
     default: return fail(NSM_ERR_ARG, "unknown field %d", field);

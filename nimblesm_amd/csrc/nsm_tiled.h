@@ -34,7 +34,10 @@ unsigned      boundary_grid(int num_bnd);
 // arithmetic, force assembly in LDS, the step of the tile-interior nodes,
 // partials of the others.  Reaction partials go to rx.partials[tile].
 // Workgroup 0 first writes the folded reaction total of a previous step.
-void launch_element_tiled(const BlockTable<float>& bt, int64_t E_total, const TileArgs& ta, int t_begin, int t_count,
+// F and sigma are tile-blocked: component c, point q of local element l of
+// tile t (elements [e0, e0 + n)) at Fip[72 e0 + (8 c + q) n + l] and
+// sip[48 e0 + (8 c + q) n + l].
+void launch_element_tiled(const BlockTable<float>& bt, const TileArgs& ta, int t_begin, int t_count,
                           const double* X, const NodeArrays& na, float* Fip, float* sip, const StepArgs& sa,
                           ReactionArgs rx, ReactionFold fold, int* status, hipStream_t s);
 // Element-kernel workgroups one CU holds at once (occupancy).

@@ -146,7 +146,7 @@ __device__ __forceinline__ void stage(const SlotIds& ids, const double (&xr)[2][
 // arithmetic always separate a load from its first use and the waits the
 // compiler places never wait for those stores.
 template <int kThreads>
-__global__ __launch_bounds__(kThreads) void k_element_tiled(const BlockTable<float> bt, int64_t E_total,
+__global__ __launch_bounds__(kThreads) void k_element_tiled(const BlockTable<float> bt,
                                                             TileArgs ta, int t_begin, int t_count,
                                                             const double* __restrict__ X, NodeArrays na,
                                                             float* __restrict__ Fip, float* __restrict__ sip,
@@ -218,7 +218,8 @@ __global__ __launch_bounds__(kThreads) void k_element_tiled(const BlockTable<flo
     const SlotIds  ids_after = load_ids<kThreads>(ta, next(next(t)));
 
     // ---- element arithmetic (every thread; see load_elem)
-    const int64_t e = e0 + min(tid, tile_table(ta.elem_off, t + 1) - e0 - 1);
+    const int ntl = tile_table(ta.elem_off, t + 1) - e0;  // elements of tile t
+    const int lcl = min(tid, ntl - 1);
     float         f[8][3];
     uint16_t      lc[8];
     lc[0] = (uint16_t)(cur.lq.x & 0xffff);
@@ -238,12 +239,18 @@ __global__ __launch_bounds__(kThreads) void k_element_tiled(const BlockTable<flo
           Xe[j][c] = sX[lc[j]][c];
           ue[j][c] = su[lc[j]][c];
         }
-      const int b     = uniform(block_of(bt, (int64_t)e0));  // tiles never straddle blocks
-      auto      store = [&](int q, const float* Fq, const float* Sq) {
+      const int b = uniform(block_of(bt, (int64_t)e0));  // tiles never straddle blocks
+      // tile-blocked F and sigma: tile t owns rows (component, point) of
+      // its n elements at [72 e0, 72 (e0 + n)) and [48 e0, 48 (e0 + n)), so
+      // each tile writes two contiguous blocks; non-temporal stores (the
+      // step never reads them back) keep node data and partials in cache
+      float* const Ft    = Fip + (int64_t)e0 * 72;
+      float* const St    = sip + (int64_t)e0 * 48;
+      auto         store = [&](int q, const float* Fq, const float* Sq) {
 #pragma unroll
-        for (int c = 0; c < 9; c++) Fip[(int64_t)(c * 8 + q) * E_total + e] = Fq[c];
+        for (int c = 0; c < 9; c++) __builtin_nontemporal_store(Fq[c], Ft + (c * 8 + q) * ntl + lcl);
 #pragma unroll
-        for (int c = 0; c < 6; c++) sip[(int64_t)(c * 8 + q) * E_total + e] = Sq[c];
+        for (int c = 0; c < 6; c++) __builtin_nontemporal_store(Sq[c], St + (c * 8 + q) * ntl + lcl);
       };
       if (!mixed::element(Xe, ue, bt.K[b], bt.G[b], store, f)) bad = 1;
     }
@@ -357,7 +364,7 @@ __global__ __launch_bounds__(kBndThreads) void k_boundary_update(TileArgs ta, No
 
 unsigned boundary_grid(int num_bnd) { return (unsigned)((num_bnd + kBndThreads - 1) / kBndThreads); }
 
-void launch_element_tiled(const BlockTable<float>& bt, int64_t E_total, const TileArgs& ta, int t_begin, int t_count,
+void launch_element_tiled(const BlockTable<float>& bt, const TileArgs& ta, int t_begin, int t_count,
                           const double* X, const NodeArrays& na, float* Fip, float* sip, const StepArgs& sa,
                           ReactionArgs rx, ReactionFold fold, int* status, hipStream_t s)
 {
@@ -365,15 +372,15 @@ void launch_element_tiled(const BlockTable<float>& bt, int64_t E_total, const Ti
   const unsigned grid = (unsigned)std::min(t_count, std::max(ta.resident_groups, 1));
   switch (ta.tile_elems) {
     case 64:
-      hipLaunchKernelGGL(k_element_tiled<64>, dim3(grid), dim3(64), 0, s, bt, E_total, ta, t_begin, t_count, X, na,
+      hipLaunchKernelGGL(k_element_tiled<64>, dim3(grid), dim3(64), 0, s, bt, ta, t_begin, t_count, X, na,
                          Fip, sip, sa, rx, fold, status);
       break;
     case 128:
-      hipLaunchKernelGGL(k_element_tiled<128>, dim3(grid), dim3(128), 0, s, bt, E_total, ta, t_begin, t_count, X,
+      hipLaunchKernelGGL(k_element_tiled<128>, dim3(grid), dim3(128), 0, s, bt, ta, t_begin, t_count, X,
                          na, Fip, sip, sa, rx, fold, status);
       break;
     default:
-      hipLaunchKernelGGL(k_element_tiled<256>, dim3(grid), dim3(256), 0, s, bt, E_total, ta, t_begin, t_count, X,
+      hipLaunchKernelGGL(k_element_tiled<256>, dim3(grid), dim3(256), 0, s, bt, ta, t_begin, t_count, X,
                          na, Fip, sip, sa, rx, fold, status);
   }
 }
