@@ -610,7 +610,7 @@ int setup_tiles(nsm_model* md)
       (st = upload(md, std::vector<int32_t>(ts.num_colors.begin(), ts.num_colors.end()), &ta.num_colors)) ||
       (st = upload(md, ts.bnd_ptr, &ta.bnd_ptr)))
     return st;
-  double* partials = nullptr;
+  float* partials = nullptr;
   if ((st = dalloc(&partials, std::max<size_t>(3 * ts.bnd_ent.size(), 1), &md->device_bytes))) return st;
   md->tile_bufs.push_back((void*)partials);
   ta.partials = partials;

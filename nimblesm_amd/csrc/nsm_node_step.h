@@ -60,6 +60,15 @@ __device__ __forceinline__ void store3(double* p, int64_t i, const double r[3])
   p[3 * i + 2] = r[2];
 }
 
+// store of a value the step does not read back (the internal force):
+// non-temporal, so it does not displace node data other launches reuse
+__device__ __forceinline__ void store3_stream(double* p, int64_t i, const double r[3])
+{
+  __builtin_nontemporal_store(r[0], p + 3 * i);
+  __builtin_nontemporal_store(r[1], p + 3 * i + 1);
+  __builtin_nontemporal_store(r[2], p + 3 * i + 2);
+}
+
 // Node i with assembled internal force f, current displacement u, velocity v
 // and lumped mass m: store f, then a = (1/m)(f + f_ext) with f_ext = 0, second
 // half kick, BCs on output steps, and (fused) the next step's half kick, BC,
@@ -69,7 +78,7 @@ __device__ __forceinline__ void node_update(const NodeArrays& na, const BcEntry*
                                             const StepArgs& sa)
 {
 #pragma clang fp contract(off)
-  store3(na.f, i, f);
+  store3_stream(na.f, i, f);
   if (!sa.update_v) return;
   const int    code     = flags & 0x7f;
   const double oneOverM = 1.0 / m;
@@ -118,7 +127,7 @@ __device__ __forceinline__ void node_step(const NodeArrays& na, int64_t i, int f
                                           double u[3], const StepArgs& sa)
 {
   if (!sa.update_v) {
-    store3(na.f, i, f);
+    store3_stream(na.f, i, f);
     return;
   }
   double v[3];

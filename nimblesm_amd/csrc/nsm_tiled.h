@@ -1,8 +1,10 @@
 // nsm_tiled.h -- launchers of the tile-local mixed-precision step (internal).
 //
-// Device arrays of a TileSet (nsm_tiles.h) plus the FP64 partial forces of
-// the tile-node slots that are not interior to their tile, stored by node
-// (the partials of one node are contiguous, in ascending tile order).
+// Device arrays of a TileSet (nsm_tiles.h) plus the partial forces of the
+// tile-node slots that are not interior to their tile, stored by node (the
+// partials of one node are contiguous, in ascending tile order).  A partial
+// is summed in FP64 inside its tile and stored in FP32, the precision of the
+// element forces it adds up.
 #pragma once
 
 #include "nsm_kernels.h"
@@ -23,7 +25,7 @@ struct TileArgs {
   const int32_t*  num_colors;  // [T]
   int             bnd_base;    // device id of boundary node 0 (boundary node b is bnd_base + b)
   const int32_t*  bnd_ptr;     // [B+1]
-  double*         partials;    // [partials][3]; node b's are [bnd_ptr[b], bnd_ptr[b+1])
+  float*          partials;    // [partials][3]; node b's are [bnd_ptr[b], bnd_ptr[b+1])
 };
 
 // boundary nodes per workgroup of the boundary kernel (one per thread)

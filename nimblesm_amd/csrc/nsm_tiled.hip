@@ -6,7 +6,7 @@
 // writes F and sigma, and its 24 node forces are added into FP64 LDS
 // accumulators colour by colour (fixed order).  Tile-interior nodes then take
 // their whole central-difference update here; every other node gets this
-// tile's partial sum.  k_boundary_update adds the partials of the remaining
+// tile's partial sum (FP64 in LDS, stored in FP32).  k_boundary_update adds the partials of the remaining
 // nodes in ascending tile order and updates them.  Each kernel also leaves one
 // reaction partial per workgroup.
 //
@@ -293,7 +293,10 @@ __global__ __launch_bounds__(kThreads) void k_element_tiled(const BlockTable<flo
         double un[3] = {su[ln][0], su[ln][1], su[ln][2]};
         node_update(na, sbc, ids.gid[k], ns.flg[k], fn, un, ns.v[k], ns.m[k], sa);
       } else {
-        store3(ta.partials, (int64_t)ids.dst[k], fn);
+        float* const pp = ta.partials + 3 * (int64_t)ids.dst[k];
+        pp[0]           = (float)fn[0];
+        pp[1]           = (float)fn[1];
+        pp[2]           = (float)fn[2];
       }
     }
     if (rx.coord >= 0) {  // uniform across the grid
@@ -335,22 +338,24 @@ __global__ __launch_bounds__(kBndThreads) void k_boundary_update(TileArgs ta, No
     // up to 8 partials are loaded at once, then added in ascending order
     // (the same order as one at a time); more than 8 continue one by one
     constexpr int kBatch = 8;
-    double        pv[kBatch][3];
+    float         pv[kBatch][3];
 #pragma unroll
     for (int i = 0; i < kBatch; i++)
-      if (p0 + i < p1) load3(ta.partials, (int64_t)p0 + i, pv[i]);
+      if (p0 + i < p1)
+#pragma unroll
+        for (int c = 0; c < 3; c++) pv[i][c] = ta.partials[3 * ((int64_t)p0 + i) + c];
     double f[3] = {0.0, 0.0, 0.0};
 #pragma unroll
     for (int i = 0; i < kBatch; i++)
       if (p0 + i < p1) {
-        f[0] += pv[i][0];
-        f[1] += pv[i][1];
-        f[2] += pv[i][2];
+        f[0] += (double)pv[i][0];
+        f[1] += (double)pv[i][1];
+        f[2] += (double)pv[i][2];
       }
     for (int64_t p = (int64_t)p0 + kBatch; p < p1; p++) {
-      f[0] += ta.partials[3 * p];
-      f[1] += ta.partials[3 * p + 1];
-      f[2] += ta.partials[3 * p + 2];
+      f[0] += (double)ta.partials[3 * p];
+      f[1] += (double)ta.partials[3 * p + 1];
+      f[2] += (double)ta.partials[3 * p + 2];
     }
     if (rx.coord >= 0 && (flags & kReactionFlag)) rxs = f[rx.coord];
     node_update(na, na.bc_table, i, flags, f, u, v, m, sa);

@@ -4,8 +4,9 @@ and against the element-slot + node-gather mixed path.
 Stated bounds (norm-wise relative to the FP64 oracle): one evaluation of the
 internal force <= 1e-5, F <= 1e-6, sigma <= 1e-5; explicit runs: displacement
 <= 1e-6, velocity, internal force and reaction history <= 1e-5.  Between the
-tiled and the slot mixed paths only the FP64 summation order of the same FP32
-element forces differs: internal force within 1e-12 for one evaluation.
+tiled and the slot mixed paths the same FP32 element forces are summed in
+another order, and the tiled path stores the partial sums of nodes shared
+between tiles in FP32: internal force within 1e-6 for one evaluation.
 """
 import numpy as np
 import pytest
@@ -104,7 +105,8 @@ def test_tiled_internal_force_against_oracle_and_slots(hip_device, monkeypatch):
     print(f"tiled vs oracle: f {rel_err(ft, f):.2e} F {rel_err(Ft, Fip):.2e} sigma {rel_err(st, sip):.2e}")
     assert rel_err(ft, f) <= 1e-5 and rel_err(Ft, Fip) <= 1e-6 and rel_err(st, sip) <= 1e-5
     fs, Fs, ss = out["slots"]
-    assert rel_err(ft, fs) <= 1e-12
+    print(f"tiled vs slots: f {rel_err(ft, fs):.2e}")
+    assert rel_err(ft, fs) <= 1e-6
     assert rel_err(Ft, Fs) <= 1e-6 and rel_err(st, ss) <= 1e-6
 
 
