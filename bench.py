@@ -8,7 +8,7 @@ CSR node gather), and the top-face reaction sum.  Inputs are resident in HBM
 before timing starts.
 
     python bench.py [--gpus 1] [--steps 500] [--warmup 20] [--precision mixed|fp64]
-                    [--workload cube128_2blk] [--cpu-steps 8] [--no-cpu] [--uq NE,NA]
+                    [--workload cube128_2blk] [--cpu-steps 40] [--no-cpu] [--uq NE,NA]
 
 Prints one JSON line.  `value` is element-steps/s of the timed K steps.
 `roofline` covers the dominant (element) kernel: its algorithmic bytes per
@@ -138,7 +138,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--precision", default="mixed", choices=["mixed", "fp64"])
     ap.add_argument("--workload", default="cube128_2blk")
-    ap.add_argument("--cpu-steps", type=int, default=8)
+    ap.add_argument("--cpu-steps", type=int, default=40)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--uq", default="", metavar="NE,NA", help="add NE exact and NA approximate UQ samples")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
