@@ -36,6 +36,12 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
+# MI355X vector peaks (TFLOP/s): FP32 (packed), FP64
+VALU_PEAK_TFLOPS = {"mixed": 157.3, "fp64": 78.6}
+# FP operations per element-step of the reference's algorithm (FMA = 2):
+# 1.23k per Gauss point (Jacobians, inverses, F, polar decomposition,
+# Neo-Hookean, dN/dx, force contraction) x 8
+COUNTED_FLOPS_PER_ELEMENT_STEP = 9840
 
 
 MAX_BLOCKS_PER_LAUNCH = 8  # element blocks sharing one element-kernel launch
@@ -108,8 +114,23 @@ def load_traffic(workload, precision):
         return json.load(f).get("bytes_per_launch")
 
 
-def cpu_baseline(prob, steps):
-    """FP64 CPU oracle, OpenMP over elements + deterministic CSR node gather."""
+def _cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(prob, steps, runs=5):
+    """FP64 CPU oracle, OpenMP over elements + deterministic CSR node gather:
+    median of `runs` runs of steps/runs steps after one warm-up step, and one
+    step of the serial (single-thread) oracle."""
+    import statistics
+
     from oracle import oracle as O  # the checker: only this leg of bench.py touches it
     from tests.helpers import oracle_model_from_problem
 
@@ -118,16 +139,30 @@ def cpu_baseline(prob, steps):
     dt = prob.dt_factor * om.critical_dt()
     run = O.ExplicitRun(om, prob.num_steps * dt, prob.num_steps, 0, threads=threads)
     run.advance(1)  # warm-up (page-in, OpenMP pool)
+    per = max(steps // runs, 1)
+    times = []
+    for _ in range(runs):
+        t0 = time.perf_counter()
+        run.advance(per)
+        times.append(time.perf_counter() - t0)
+    med = statistics.median(times)
+    serial = O.ExplicitRun(om, prob.num_steps * dt, prob.num_steps, 0, threads=1)
     t0 = time.perf_counter()
-    run.advance(steps)
-    el = time.perf_counter() - t0
+    serial.advance(1)
+    ts = time.perf_counter() - t0
     return {
-        "value": prob.num_elem * steps / el,
+        "value": prob.num_elem * per / med,
         "unit": "element-steps/s",
         "cores": threads,
         "kind": "port",
-        "sample": f"{steps} FP64 steps of {prob.name} ({prob.num_elem} elements) after 1 warm-up step, "
-                  f"{threads} OpenMP threads, {el:.1f} s",
+        "sample": f"median of {runs} runs of {per} FP64 steps of {prob.name} ({prob.num_elem} elements) after 1 "
+                  f"warm-up step, {threads} OpenMP threads ({sum(times):.1f} s in all); serial: 1 step on one "
+                  f"thread ({ts:.1f} s)",
+        "runs_s": times,
+        "serial_value": prob.num_elem / ts,
+        "cpu_model": _cpu_model(),
+        "nproc": os.cpu_count(),
+        "omp_num_threads": os.environ.get("OMP_NUM_THREADS"),
     }
 
 
@@ -215,6 +250,9 @@ def main():
     ms_per_step = elapsed * 1e3 / args.steps
     value = prob.num_elem * args.steps / elapsed
     R = model.GetReactionHistory(0, args.steps)
+    # compulsory floor: connectivity, X, u, v, m read and u, v written once
+    # (no integration-point state, no internal-force write)
+    floor_bytes = prob.num_elem * 32 + prob.num_nodes * 128
 
     out = {
         "metric": "element-steps/s + % HBM roofline",
@@ -260,6 +298,14 @@ def main():
             "frac": step_bytes / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS,
         },
         "reaction_final": float(R[-1]),
+        "secondary": {
+            "counted_flops_per_element_step": COUNTED_FLOPS_PER_ELEMENT_STEP,
+            "valu_peak_tflops": VALU_PEAK_TFLOPS[args.precision],
+            "valu_frac": COUNTED_FLOPS_PER_ELEMENT_STEP * value / (VALU_PEAK_TFLOPS[args.precision] * 1e12),
+            "compulsory_floor_bytes_per_element": floor_bytes / prob.num_elem,
+            "compulsory_floor_frac": floor_bytes / prob.num_elem * value / (HBM_PEAK_GBS * 1e9),
+            "element_measured_hbm_GBs": traffic / (el_ms * 1e-3) / 1e9 if traffic and el_ms > 0 else None,
+        },
     }
     if args.uq:
         cb = closure_bytes(prob.num_nodes, ne, na)

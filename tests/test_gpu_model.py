@@ -194,6 +194,29 @@ def test_large_cube_properties(hip_device):
     assert rel_err(res["mixed"][2], res["fp64"][2]) <= 1e-5
 
 
+def test_largest_cube_mixed_against_fp64_gpu(hip_device):
+    """256^3, four z-layered blocks, the full 200 steps: mixed precision
+    against an FP64 run on the device, with the size-independent checks."""
+    prob = N.workload("cube256_4blk")
+    steps = prob.num_steps
+    res = {}
+    for prec in ("fp64", "mixed"):
+        print(f"cube256_4blk {prec}: {steps} steps", flush=True)
+        m, integ = run_gpu_problem(prob, steps, prec)
+        f = m.GetVectorNodeData("internal_force")
+        u = m.GetVectorNodeData("displacement")
+        R = m.GetReactionHistory(0, steps)
+        assert np.abs(f.sum(axis=0)).max() <= 1e-9 * np.abs(f).sum(axis=0).max()
+        assert np.allclose(u[prob.top, 2], integ.time_current * prob.v_top, rtol=1e-12)
+        assert np.all(u[prob.bottom] == 0.0)
+        res[prec] = (u, f, R)
+        m.close()
+        del m
+    eu, ef, eR = (rel_err(res["mixed"][i], res["fp64"][i]) for i in range(3))
+    print(f"cube256_4blk mixed vs fp64 GPU after {steps} steps: u {eu:.2e} f {ef:.2e} R {eR:.2e}")
+    assert eu <= 1e-6 and ef <= 1e-5 and eR <= 1e-5
+
+
 def test_reaction_over_scattered_nodes(hip_device):
     """Reaction sum over an arbitrary node set (many node tiles, some nodes
     with zero force in early steps) against the oracle's serial sum."""
