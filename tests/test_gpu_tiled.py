@@ -157,6 +157,28 @@ def test_tiled_run_with_ten_blocks(hip_device):
     assert eF <= 1e-6 and es <= 1e-5
 
 
+@pytest.mark.parametrize("tile_elems", [64, 128])
+def test_smaller_tiles_against_oracle(hip_device, monkeypatch, tile_elems):
+    """NSM_TILE_ELEMS=64 / 128 (one or two waves per tile): tile sizes, a run
+    and the tile-blocked F / sigma against the oracle."""
+    monkeypatch.setenv("NSM_TILE_ELEMS", str(tile_elems))
+    n = 10
+    X, conn = N.jittered_cube(n, 9, 0.1)
+    mats = [N.NeohookeanMaterial.from_young(7800, 200e9, 0.3), N.NeohookeanMaterial.from_young(2700, 70e9, 0.33)]
+    steps = 40
+    m, run = run_both(X, conn, [0, n ** 3 // 2, n ** 3], mats, steps)
+    info = m.AssemblyInfo()
+    assert info["tiled"] and info["num_tiles"] >= n ** 3 // tile_elems
+    eu = rel_err(m.GetVectorNodeData("displacement"), run.u)
+    ef = rel_err(m.GetVectorNodeData("internal_force"), run.f)
+    eR = rel_err(m.GetReactionHistory(0, steps), run.reaction)
+    _, Fo, so = run.model.internal_force(run.u, want_ip=True)
+    eF = rel_err(m.GetIntegrationPointData("deformation_gradient"), Fo)
+    es = rel_err(m.GetIntegrationPointData("stress"), so)
+    print(f"{tile_elems}-element tiles vs oracle: u {eu:.2e} f {ef:.2e} R {eR:.2e} F {eF:.2e} sigma {es:.2e}")
+    assert eu <= 1e-6 and ef <= 1e-5 and eR <= 1e-5 and eF <= 1e-6 and es <= 1e-5
+
+
 def test_tiled_matches_slots_over_a_run(hip_device, monkeypatch):
     prob = N.CubeProblem(n=16, materials=[N.problems.STEEL, N.problems.ALUMINIUM], num_steps=200)
     res = {}
