@@ -590,7 +590,7 @@ int setup_tiles(nsm_model* md)
   nsm::TileArgs&      ta = md->tile_args;
   // the element kernel runs one thread per tile element and stages at most
   // two tile nodes per thread
-  if (ts.tile_elems != 64 && ts.tile_elems != 128 && ts.tile_elems != 256)
+  if (ts.tile_elems != 64 && ts.tile_elems != 128 && ts.tile_elems != 256 && ts.tile_elems != 512)
     return fail(NSM_ERR_ARG, "unsupported tile size");
   for (int t = 0; t < ts.num_tiles; t++)
     if (ts.elem_off[(size_t)t + 1] - ts.elem_off[(size_t)t] > ts.tile_elems ||
@@ -818,11 +818,12 @@ int nsm_model_create(int64_t num_nodes, const double* reference_coordinates, int
     const char* mode = std::getenv("NSM_MIXED_ASSEMBLY");
     if (!(mode && std::strcmp(mode, "slots") == 0)) {
       // NSM_TILE_ELEMS=128 or 64 selects smaller tiles (two waves / one wave
-      // per workgroup)
+      // per workgroup), 512 larger ones (eight waves, one workgroup per CU)
       const char* te         = std::getenv("NSM_TILE_ELEMS");
       const int   tile_elems = !te                          ? nsm::kTileElems
                                : std::strcmp(te, "128") == 0 ? 128
                                : std::strcmp(te, "64") == 0  ? 64
+                               : std::strcmp(te, "512") == 0 ? 512
                                                              : nsm::kTileElems;
       std::string err;
       md->tiled = nsm::build_tiles(num_nodes, reference_coordinates, conn, off, tile_elems, md->tiles, &err);

@@ -102,13 +102,14 @@ __device__ __forceinline__ void node_update(const NodeArrays& na, const BcEntry*
 
 // Reaction total -sum_k partials[tiles[k]] of one step, summed in 256
 // interleaved lanes (lane v takes k = v, v + 256, ...) and then a fixed tree,
-// so the result is the same for any workgroup size kThreads dividing 256.
+// so the result is the same for any workgroup size kThreads dividing 256 or
+// divisible by it (threads past 256 then only take part in the barriers).
 // red: 256 doubles of LDS.  Valid in thread 0.
 template <int kThreads>
 __device__ __forceinline__ double reaction_total(const double* partials, const int* tiles, int n, double* red)
 {
   constexpr int kLanes = 256;
-  static_assert(kLanes % kThreads == 0, "workgroup size must divide 256");
+  static_assert(kLanes % kThreads == 0 || kThreads % kLanes == 0, "workgroup size and 256 must divide");
   for (int v = threadIdx.x; v < kLanes; v += kThreads) {
     double s = 0.0;
     for (int k = v; k < n; k += kLanes) s += partials[tiles[k]];

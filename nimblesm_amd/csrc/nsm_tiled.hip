@@ -384,6 +384,10 @@ void launch_element_tiled(const BlockTable<float>& bt, const TileArgs& ta, int t
       hipLaunchKernelGGL(k_element_tiled<128>, dim3(grid), dim3(128), 0, s, bt, ta, t_begin, t_count, X,
                          na, Fip, sip, sa, rx, fold, status);
       break;
+    case 512:
+      hipLaunchKernelGGL(k_element_tiled<512>, dim3(grid), dim3(512), 0, s, bt, ta, t_begin, t_count, X,
+                         na, Fip, sip, sa, rx, fold, status);
+      break;
     default:
       hipLaunchKernelGGL(k_element_tiled<256>, dim3(grid), dim3(256), 0, s, bt, ta, t_begin, t_count, X,
                          na, Fip, sip, sa, rx, fold, status);
@@ -396,6 +400,7 @@ int element_tiled_groups_per_cu(int tile_elems)
   switch (tile_elems) {
     case 64: (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_element_tiled<64>, 64, 0); break;
     case 128: (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_element_tiled<128>, 128, 0); break;
+    case 512: (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_element_tiled<512>, 512, 0); break;
     default: (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_element_tiled<256>, 256, 0);
   }
   return n;

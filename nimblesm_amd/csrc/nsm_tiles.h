@@ -1,7 +1,7 @@
 // nsm_tiles.h -- element tiles with tile-local force assembly (mixed path).
 //
 // Elements of each block are put in Morton order of their centroids and cut
-// into tiles of at most tile_elems elements (256 by default, or 128) touching
+// into tiles of at most tile_elems elements (256 by default; 64, 128, 512) touching
 // at most 2*tile_elems distinct nodes.  A node whose incident elements all lie in one tile is
 // "interior" to it: that tile sums its force completely and performs its
 // central-difference update in the element kernel.  Every other node gets one

@@ -157,10 +157,10 @@ def test_tiled_run_with_ten_blocks(hip_device):
     assert eF <= 1e-6 and es <= 1e-5
 
 
-@pytest.mark.parametrize("tile_elems", [64, 128])
-def test_smaller_tiles_against_oracle(hip_device, monkeypatch, tile_elems):
-    """NSM_TILE_ELEMS=64 / 128 (one or two waves per tile): tile sizes, a run
-    and the tile-blocked F / sigma against the oracle."""
+@pytest.mark.parametrize("tile_elems", [64, 128, 512])
+def test_other_tile_sizes_against_oracle(hip_device, monkeypatch, tile_elems):
+    """NSM_TILE_ELEMS=64 / 128 / 512 (1, 2 or 8 waves per tile): tile sizes, a
+    run and the tile-blocked F / sigma against the oracle."""
     monkeypatch.setenv("NSM_TILE_ELEMS", str(tile_elems))
     n = 10
     X, conn = N.jittered_cube(n, 9, 0.1)
