@@ -204,12 +204,14 @@ __global__ __launch_bounds__(kThreads) void k_element_fp64(const BlockTable<doub
     jacobians(k, lds.nodes.x[el], lds.nodes.u[el], a, b);
     bad |= invert3x3(b, bi) <= 0.0;
     def_grad(a, bi, F);
-    // element-fastest [comp][ipt][E] layout
+    // element-fastest [comp][ipt][E] layout (no stores when Fip is null)
+    if (Fip)
 #pragma unroll
-    for (int i = 0; i < 9; i++) Fip[(int64_t)(i * 8 + k) * E_total + e] = F[i];
+      for (int i = 0; i < 9; i++) Fip[(int64_t)(i * 8 + k) * E_total + e] = F[i];
     neohookean_stress(K, G, F, sig);
+    if (sip)
 #pragma unroll
-    for (int i = 0; i < 6; i++) sip[(int64_t)(i * 8 + k) * E_total + e] = sig[i];
+      for (int i = 0; i < 6; i++) sip[(int64_t)(i * 8 + k) * E_total + e] = sig[i];
     // the force routine rebuilds a from the same x = X + u in the same order
     ainv[9] = invert3x3(a, ainv);
     bad |= ainv[9] <= 0.0;

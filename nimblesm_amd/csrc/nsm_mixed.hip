@@ -49,8 +49,10 @@ __global__ __launch_bounds__(kThreads) NSM_MIXED_WAVES_ATTR void k_element_mixed
     ue[j][1]        = u[3 * n + 1];
     ue[j][2]        = u[3 * n + 2];
   }
-  // F and sigma are not read again by the step: non-temporal stores
+  // F and sigma are not read again by the step: non-temporal stores (none
+  // when Fip is null)
   auto store = [&](int q, const float* Fq, const float* Sq) {
+    if (!Fip) return;
 #pragma unroll
     for (int c = 0; c < 9; c++) __builtin_nontemporal_store(Fq[c], Fip + (int64_t)(c * 8 + q) * E_total + e);
 #pragma unroll

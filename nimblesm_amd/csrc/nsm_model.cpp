@@ -398,6 +398,11 @@ int check_status(nsm_model* md)
   return NSM_OK;
 }
 
+// F / sigma destination of an element launch: the model's arrays for the
+// nominal trajectory, none for UQ samples (the reference keeps no element
+// data for them, src/uq/nimble_uq_block.cc:101-163)
+void* ip_out(const nsm_model*, void* p, int sample) { return sample < 0 ? p : nullptr; }
+
 nsm::BlockTable<float> float_table(const nsm_model* md, int b0, int nb, int sample)
 {
   nsm::BlockTable<float> bt{};
@@ -429,11 +434,12 @@ int launch_elements(nsm_model* md, int sample)
         bt.K[b] = md->bulk(sample, b0 + b);
         bt.G[b] = md->shear(sample, b0 + b);
       }
-      nsm::launch_element_fp64(bt, md->E, md->conn, md->X, u, (double*)md->Fip, (double*)md->sip,
-                               (double*)md->slots, md->status, md->stream);
+      nsm::launch_element_fp64(bt, md->E, md->conn, md->X, u, (double*)ip_out(md, md->Fip, sample),
+                               (double*)ip_out(md, md->sip, sample), (double*)md->slots, md->status, md->stream);
     } else {
-      nsm::launch_element_mixed(float_table(md, b0, nb, sample), md->E, md->conn, md->X, u, (float*)md->Fip,
-                                (float*)md->sip, (float*)md->slots, md->status, md->stream);
+      nsm::launch_element_mixed(float_table(md, b0, nb, sample), md->E, md->conn, md->X, u,
+                                (float*)ip_out(md, md->Fip, sample), (float*)ip_out(md, md->sip, sample),
+                                (float*)md->slots, md->status, md->stream);
     }
     NSM_LAUNCHED();
     if ((st = time_end(md, ti))) return st;
@@ -515,7 +521,8 @@ int launch_tiled(nsm_model* md, const nsm::StepArgs& sa, int64_t step, int64_t f
     size_t ti;
     if ((st = time_begin(md, NSM_KERNEL_ELEMENT, &ti))) return st;
     nsm::launch_element_tiled(float_table(md, b0, nb, sample), md->tile_args, t0, t1 - t0, md->X, na,
-                              (float*)md->Fip, (float*)md->sip, sa, rx, fold, md->status, md->stream);
+                              (float*)ip_out(md, md->Fip, sample), (float*)ip_out(md, md->sip, sample), sa, rx,
+                              fold, md->status, md->stream);
     NSM_LAUNCHED();
     if ((st = time_end(md, ti))) return st;
     fold.n = 0;
@@ -555,12 +562,11 @@ int launch_closure(nsm_model* md, const nsm::StepArgs& sa)
   return time_end(md, ti);
 }
 
-// One step of every trajectory.  Exact UQ samples run first: their element
-// launches write F and sigma into the nominal arrays (the reference keeps no
-// element data for them, src/uq/nimble_uq_block.cc:101-163), which the
-// nominal launch then overwrites.  Samples are independent of each other and
-// of the nominal trajectory until the closure, which reads this step's
-// nominal and exact forces.
+// One step of every trajectory: exact UQ samples (their element launches
+// store no F or sigma), the nominal trajectory, then the closure of the
+// approximate samples, which reads this step's nominal and exact forces.
+// Samples are independent of each other and of the nominal trajectory until
+// the closure.
 int launch_step(nsm_model* md, const nsm::StepArgs& sa, int64_t step, int64_t fold_step, int64_t* pending)
 {
   int     st     = NSM_OK;
@@ -614,6 +620,10 @@ int setup_tiles(nsm_model* md)
   if ((st = dalloc(&partials, std::max<size_t>(3 * ts.bnd_ent.size(), 1), &md->device_bytes))) return st;
   md->tile_bufs.push_back((void*)partials);
   ta.partials = partials;
+  float* sink = nullptr;
+  if ((st = dalloc(&sink, (size_t)8 * 120 * ts.tile_elems, &md->device_bytes))) return st;
+  md->tile_bufs.push_back((void*)sink);
+  ta.ip_sink = sink;
   md->blk_tile_off.assign((size_t)md->num_blocks + 1, 0);
   for (int b = 0; b <= md->num_blocks; b++)
     md->blk_tile_off[(size_t)b] =

@@ -26,6 +26,7 @@ struct TileArgs {
   int             bnd_base;    // device id of boundary node 0 (boundary node b is bnd_base + b)
   const int32_t*  bnd_ptr;     // [B+1]
   float*          partials;    // [partials][3]; node b's are [bnd_ptr[b], bnd_ptr[b+1])
+  float*          ip_sink;     // [8][120][tile_elems]: F / sigma stores of launches that keep none
 };
 
 // boundary nodes per workgroup of the boundary kernel (one per thread)
@@ -38,7 +39,8 @@ unsigned      boundary_grid(int num_bnd);
 // Workgroup 0 first writes the folded reaction total of a previous step.
 // F and sigma are tile-blocked: component c, point q of local element l of
 // tile t (elements [e0, e0 + n)) at Fip[72 e0 + (8 c + q) n + l] and
-// sip[48 e0 + (8 c + q) n + l].
+// sip[48 e0 + (8 c + q) n + l]; with Fip = sip = nullptr they go to
+// ta.ip_sink (overwritten, cache resident) instead.
 void launch_element_tiled(const BlockTable<float>& bt, const TileArgs& ta, int t_begin, int t_count,
                           const double* X, const NodeArrays& na, float* Fip, float* sip, const StepArgs& sa,
                           ReactionArgs rx, ReactionFold fold, int* status, hipStream_t s);

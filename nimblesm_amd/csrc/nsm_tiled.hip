@@ -144,8 +144,12 @@ __device__ __forceinline__ void stage(const SlotIds& ids, const double (&xr)[2][
 // Inside the loop every vector load and every F / sigma store is executed on
 // every path (indices clamped), so at least the 120 stores of the element
 // arithmetic always separate a load from its first use and the waits the
-// compiler places never wait for those stores.
-template <int kThreads>
+// compiler places never wait for those stores.  kStoreIp = false (off-nominal
+// UQ samples, which keep no element data) sends the same stores to a small
+// sink that stays in L2 instead of F and sigma: without them the compiler
+// schedules the element arithmetic with more registers than a wave pair
+// has, and the kernel is no faster for the bytes it saves.
+template <int kThreads, bool kStoreIp>
 __global__ __launch_bounds__(kThreads) void k_element_tiled(const BlockTable<float> bt,
                                                             TileArgs ta, int t_begin, int t_count,
                                                             const double* __restrict__ X, NodeArrays na,
@@ -244,13 +248,20 @@ __global__ __launch_bounds__(kThreads) void k_element_tiled(const BlockTable<flo
       // its n elements at [72 e0, 72 (e0 + n)) and [48 e0, 48 (e0 + n)), so
       // each tile writes two contiguous blocks; non-temporal stores (the
       // step never reads them back) keep node data and partials in cache
-      float* const Ft    = Fip + (int64_t)e0 * 72;
-      float* const St    = sip + (int64_t)e0 * 48;
+      float* const Ft    = kStoreIp ? Fip + (int64_t)e0 * 72 : ta.ip_sink + (t & 7) * 120 * ntl;
+      float* const St    = kStoreIp ? sip + (int64_t)e0 * 48 : Ft + 72 * ntl;
       auto         store = [&](int q, const float* Fq, const float* Sq) {
+        if constexpr (kStoreIp) {
 #pragma unroll
-        for (int c = 0; c < 9; c++) __builtin_nontemporal_store(Fq[c], Ft + (c * 8 + q) * ntl + lcl);
+          for (int c = 0; c < 9; c++) __builtin_nontemporal_store(Fq[c], Ft + (c * 8 + q) * ntl + lcl);
 #pragma unroll
-        for (int c = 0; c < 6; c++) __builtin_nontemporal_store(Sq[c], St + (c * 8 + q) * ntl + lcl);
+          for (int c = 0; c < 6; c++) __builtin_nontemporal_store(Sq[c], St + (c * 8 + q) * ntl + lcl);
+        } else {  // the sink is rewritten every 8 tiles: keep it in L2
+#pragma unroll
+          for (int c = 0; c < 9; c++) Ft[(c * 8 + q) * ntl + lcl] = Fq[c];
+#pragma unroll
+          for (int c = 0; c < 6; c++) St[(c * 8 + q) * ntl + lcl] = Sq[c];
+        }
       };
       if (!mixed::element(Xe, ue, bt.K[b], bt.G[b], store, f)) bad = 1;
     }
@@ -369,6 +380,19 @@ __global__ __launch_bounds__(kBndThreads) void k_boundary_update(TileArgs ta, No
 
 unsigned boundary_grid(int num_bnd) { return (unsigned)((num_bnd + kBndThreads - 1) / kBndThreads); }
 
+template <int kThreads>
+void launch_tiles(unsigned grid, const BlockTable<float>& bt, const TileArgs& ta, int t_begin, int t_count,
+                  const double* X, const NodeArrays& na, float* Fip, float* sip, const StepArgs& sa, ReactionArgs rx,
+                  ReactionFold fold, int* status, hipStream_t s)
+{
+  if (Fip)
+    hipLaunchKernelGGL((k_element_tiled<kThreads, true>), dim3(grid), dim3(kThreads), 0, s, bt, ta, t_begin,
+                       t_count, X, na, Fip, sip, sa, rx, fold, status);
+  else
+    hipLaunchKernelGGL((k_element_tiled<kThreads, false>), dim3(grid), dim3(kThreads), 0, s, bt, ta, t_begin,
+                       t_count, X, na, Fip, sip, sa, rx, fold, status);
+}
+
 void launch_element_tiled(const BlockTable<float>& bt, const TileArgs& ta, int t_begin, int t_count,
                           const double* X, const NodeArrays& na, float* Fip, float* sip, const StepArgs& sa,
                           ReactionArgs rx, ReactionFold fold, int* status, hipStream_t s)
@@ -376,21 +400,10 @@ void launch_element_tiled(const BlockTable<float>& bt, const TileArgs& ta, int t
   if (t_count <= 0) return;
   const unsigned grid = (unsigned)std::min(t_count, std::max(ta.resident_groups, 1));
   switch (ta.tile_elems) {
-    case 64:
-      hipLaunchKernelGGL(k_element_tiled<64>, dim3(grid), dim3(64), 0, s, bt, ta, t_begin, t_count, X, na,
-                         Fip, sip, sa, rx, fold, status);
-      break;
-    case 128:
-      hipLaunchKernelGGL(k_element_tiled<128>, dim3(grid), dim3(128), 0, s, bt, ta, t_begin, t_count, X,
-                         na, Fip, sip, sa, rx, fold, status);
-      break;
-    case 512:
-      hipLaunchKernelGGL(k_element_tiled<512>, dim3(grid), dim3(512), 0, s, bt, ta, t_begin, t_count, X,
-                         na, Fip, sip, sa, rx, fold, status);
-      break;
-    default:
-      hipLaunchKernelGGL(k_element_tiled<256>, dim3(grid), dim3(256), 0, s, bt, ta, t_begin, t_count, X,
-                         na, Fip, sip, sa, rx, fold, status);
+    case 64: launch_tiles<64>(grid, bt, ta, t_begin, t_count, X, na, Fip, sip, sa, rx, fold, status, s); break;
+    case 128: launch_tiles<128>(grid, bt, ta, t_begin, t_count, X, na, Fip, sip, sa, rx, fold, status, s); break;
+    case 512: launch_tiles<512>(grid, bt, ta, t_begin, t_count, X, na, Fip, sip, sa, rx, fold, status, s); break;
+    default: launch_tiles<256>(grid, bt, ta, t_begin, t_count, X, na, Fip, sip, sa, rx, fold, status, s);
   }
 }
 
@@ -398,10 +411,10 @@ int element_tiled_groups_per_cu(int tile_elems)
 {
   int n = 0;
   switch (tile_elems) {
-    case 64: (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_element_tiled<64>, 64, 0); break;
-    case 128: (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_element_tiled<128>, 128, 0); break;
-    case 512: (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_element_tiled<512>, 512, 0); break;
-    default: (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_element_tiled<256>, 256, 0);
+    case 64: (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_element_tiled<64, true>, 64, 0); break;
+    case 128: (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_element_tiled<128, true>, 128, 0); break;
+    case 512: (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_element_tiled<512, true>, 512, 0); break;
+    default: (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_element_tiled<256, true>, 256, 0);
   }
   return n;
 }
