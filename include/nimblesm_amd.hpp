@@ -173,6 +173,20 @@ class ModelData {
     check(nsm_model_get_reaction_history(h_, first, count, out.data()));
     return out;
   }
+  // off-nominal ensemble: K, G [num_exact][num_blocks] (ascending block
+  // id), closure coefficients [num_approx][num_exact + 1]
+  void SetUqSamples(int num_exact, const std::vector<double>& K, const std::vector<double>& G, int num_approx,
+                    const std::vector<double>& coefficients)
+  {
+    check(nsm_model_set_uq_samples(h_, num_exact, K.data(), G.data(), num_approx, coefficients.data()));
+  }
+  // node vector of UQ sample `sample` (exact samples first)
+  std::vector<double> GetUqNodeData(int sample, nsm_field f)
+  {
+    std::vector<double> out((size_t)(3 * num_nodes_));
+    check(nsm_model_get_uq_field(h_, sample, f, out.data()));
+    return out;
+  }
   nsm_model* handle() { return h_; }
   int64_t    num_nodes() const { return num_nodes_; }
   int64_t    num_elem() const { return num_elem_; }

@@ -252,3 +252,26 @@ def test_sampled_reaction_history(hip_device):
         assert np.array_equal(s, O.sample_history(R, 64))
         assert max_rel_err(s, ref) <= tol, (precision, max_rel_err(s, ref))
         m.close()
+
+
+@pytest.mark.parametrize("mode", ["fp64", "tiles", "slots"])
+def test_inverted_element_in_model_reports_error_and_recovers(hip_device, monkeypatch, mode):
+    """A displacement that turns elements inside out makes the model's force
+    evaluation fail with NSM_ERR_NEG_JACOBIAN (the reference's Invert3x3
+    check); the status is cleared, so a valid displacement then works."""
+    if mode != "fp64":
+        monkeypatch.setenv("NSM_MIXED_ASSEMBLY", mode)
+    prob = N.CubeProblem(n=4, materials=[N.problems.STEEL], num_steps=1)
+    m = prob.make_model("fp64" if mode == "fp64" else "mixed")
+    assert m.AssemblyInfo()["tiled"] == (mode == "tiles")
+    u = np.zeros((prob.num_nodes, 3))
+    center = 2 + 5 * (2 + 5 * 2)  # interior node (2, 2, 2)
+    u[center, 2] = -0.6  # pushed through the node below it (h = 0.25)
+    m.SetVectorNodeData("displacement", u)
+    with pytest.raises(N.NimbleError) as ei:
+        m.ComputeInternalForce()
+    assert ei.value.status == 3
+    m.SetVectorNodeData("displacement", np.zeros_like(u))
+    m.ComputeInternalForce()
+    assert np.abs(m.GetVectorNodeData("internal_force")).max() < 1e-3  # undeformed: f ~ 0
+    m.close()
