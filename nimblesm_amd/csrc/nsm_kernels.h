@@ -127,9 +127,10 @@ void launch_prekick(const NodeArrays& na, StepTimes t, hipStream_t s);
 struct StepArgs;
 // gather f from element slots, a = f/m, second half kick (+BC on output
 // steps); if fused, also the next step's first half kick, BC, drift, BC;
-// optional per-tile reaction partials.
+// optional per-tile reaction partials; workgroup 0 first sums the fold.
 template <typename SlotT>
-void launch_node_update(const NodeArrays& na, const SlotT* slots, const StepArgs& sa, ReactionArgs rx, hipStream_t s);
+void launch_node_update(const NodeArrays& na, const SlotT* slots, const StepArgs& sa, ReactionArgs rx,
+                        ReactionFold fold, hipStream_t s);
 // Off-nominal ensemble: sample s owns u, v, a, f at state + (4 s + j) stride,
 // j = 0..3 (exact samples first).  The closure kernel forms each approximate
 // sample's force f_s = c[s][0] f_nominal + sum_k c[s][k+1] f_exact_k

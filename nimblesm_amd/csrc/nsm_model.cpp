@@ -5,8 +5,9 @@
 // life; a step is a short sequence of stream-ordered launches with no host
 // round trip: FP64 runs the element kernel (force slots) then the CSR node
 // kernel; mixed precision runs the tiled element kernel (LDS assembly and the
-// update of tile-interior nodes) then the boundary-node kernel; either ends
-// with the optional reaction sum.
+// update of tile-interior nodes) then the boundary-node kernel.  A step's
+// reaction total is summed by workgroup 0 of the next step's first launch
+// (or by a separate launch after a call's last step).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -480,23 +481,36 @@ int launch_reaction(nsm_model* md, int64_t step)
   return time_end(md, ti);
 }
 
-// Node kernel of the slot path; the reaction partials of `step` are folded
-// into it (nominal trajectory only).
-int launch_node(nsm_model* md, const nsm::StepArgs& sa, int64_t step, int sample)
+// The reaction total of `fold_step` (a previous step, or -1), summed by
+// workgroup 0 of the next launch that takes it.
+nsm::ReactionFold reaction_fold(const nsm_model* md, int64_t fold_step)
+{
+  if (fold_step < 0) return nsm::ReactionFold{nullptr, nullptr, 0, nullptr};
+  return nsm::ReactionFold{reaction_partials_of(md, fold_step), md->reaction_tiles, md->num_reaction_tiles,
+                           md->reaction_hist + fold_step};
+}
+
+// Node kernel of the slot path: it writes the reaction partials of `step`
+// (nominal trajectory only; left pending, `*pending` = step when there are
+// any) and sums those of `fold_step` first.
+int launch_node(nsm_model* md, const nsm::StepArgs& sa, int64_t step, int64_t fold_step, int64_t* pending,
+                int sample)
 {
   size_t ti;
   int    st = time_begin(md, NSM_KERNEL_NODE, &ti);
   if (st) return st;
-  nsm::ReactionArgs     rx;
-  const bool            want_rx = reaction_args(md, sample < 0 ? step : -1, &rx);
-  const nsm::NodeArrays na      = md->node_arrays(sample);
+  nsm::ReactionArgs       rx;
+  const bool              want_rx = reaction_args(md, sample < 0 ? step : -1, &rx);
+  const nsm::NodeArrays   na      = md->node_arrays(sample);
+  const nsm::ReactionFold fold    = reaction_fold(md, fold_step);
   if (md->precision == NSM_FP64)
-    nsm::launch_node_update<double>(na, (const double*)md->slots, sa, rx, md->stream);
+    nsm::launch_node_update<double>(na, (const double*)md->slots, sa, rx, fold, md->stream);
   else
-    nsm::launch_node_update<float>(na, (const float*)md->slots, sa, rx, md->stream);
+    nsm::launch_node_update<float>(na, (const float*)md->slots, sa, rx, fold, md->stream);
   NSM_LAUNCHED();
   if ((st = time_end(md, ti))) return st;
-  return want_rx ? launch_reaction(md, step) : NSM_OK;
+  *pending = want_rx ? step : -1;
+  return NSM_OK;
 }
 
 // Tile-local step: element tiles (with the interior-node update), then the
@@ -509,10 +523,7 @@ int launch_tiled(nsm_model* md, const nsm::StepArgs& sa, int64_t step, int64_t f
   nsm::ReactionArgs     rx;
   const bool            want_rx = reaction_args(md, step, &rx);
   const nsm::NodeArrays na      = md->node_arrays(sample);
-  nsm::ReactionFold     fold{nullptr, nullptr, 0, nullptr};
-  if (fold_step >= 0)
-    fold = nsm::ReactionFold{reaction_partials_of(md, fold_step), md->reaction_tiles, md->num_reaction_tiles,
-                             md->reaction_hist + fold_step};
+  nsm::ReactionFold     fold    = reaction_fold(md, fold_step);
   int st = NSM_OK;
   for (int b0 = 0; b0 < md->num_blocks; b0 += nsm::kMaxLaunchBlocks) {
     const int nb = std::min(nsm::kMaxLaunchBlocks, md->num_blocks - b0);
@@ -537,16 +548,17 @@ int launch_tiled(nsm_model* md, const nsm::StepArgs& sa, int64_t step, int64_t f
 }
 
 // One trajectory's force evaluation and node update (update_v = 0: f_int
-// only).  The slot path finalizes the reaction of `step` itself; the tiled
-// path leaves it pending (see launch_tiled) and folds `fold_step` instead.
-// UQ samples (sample >= 0) pass step = fold_step = -1: no reaction.
+// only).  The reaction total of `step` is left pending (`*pending` = step
+// when there is one) and that of `fold_step` is summed by the first launch
+// that can take it.  UQ samples (sample >= 0) pass step = fold_step = -1:
+// no reaction.
 int launch_trajectory(nsm_model* md, const nsm::StepArgs& sa, int64_t step, int64_t fold_step, int64_t* pending,
                       int sample)
 {
   *pending = -1;
   if (md->tiled) return launch_tiled(md, sa, step, fold_step, pending, sample);
   int st = launch_elements(md, sample);
-  return st ? st : launch_node(md, sa, step, sample);
+  return st ? st : launch_node(md, sa, step, fold_step, pending, sample);
 }
 
 // Closure forces of the approximate UQ samples from this step's nominal and

@@ -50,11 +50,17 @@ __global__ __launch_bounds__(kNodeThreads) void k_prekick(NodeArrays na, StepTim
 // Internal-force gather and the second half of step n
 // (src/integrators/explicit_time_integrator.cc:250-270), optionally fused
 // with the first half of step n+1, plus per-tile partial reaction sums over
-// flagged nodes.
+// flagged nodes.  Workgroup 0 first writes the folded reaction total of a
+// previous step.
 template <typename SlotT>
 __global__ __launch_bounds__(kNodeThreads) void k_node_update(NodeArrays na, const SlotT* __restrict__ slots,
-                                                              StepArgs sa, ReactionArgs rx)
+                                                              StepArgs sa, ReactionArgs rx, ReactionFold fold)
 {
+  if (fold.n > 0 && blockIdx.x == 0) {  // uniform
+    __shared__ double fold_red[256];
+    const double      total = reaction_total<kNodeThreads>(fold.partials, fold.tiles, fold.n, fold_red);
+    if (threadIdx.x == 0) *fold.out = total;
+  }
   const unsigned tile    = xcd_tile(blockIdx.x, gridDim.x);
   const int64_t  i       = (int64_t)tile * kNodeThreads + threadIdx.x;
   double         contrib = 0.0;
@@ -221,16 +227,18 @@ void launch_prekick(const NodeArrays& na, StepTimes t, hipStream_t s)
 }
 
 template <typename SlotT>
-void launch_node_update(const NodeArrays& na, const SlotT* slots, const StepArgs& sa, ReactionArgs rx, hipStream_t s)
+void launch_node_update(const NodeArrays& na, const SlotT* slots, const StepArgs& sa, ReactionArgs rx,
+                        ReactionFold fold, hipStream_t s)
 {
   if (na.num_nodes <= 0) return;
   hipLaunchKernelGGL((k_node_update<SlotT>), dim3(node_grid(na.num_nodes)), dim3(kNodeThreads), 0, s, na, slots, sa,
-                     rx);
+                     rx, fold);
 }
 
 template void launch_node_update<double>(const NodeArrays&, const double*, const StepArgs&, ReactionArgs,
-                                         hipStream_t);
-template void launch_node_update<float>(const NodeArrays&, const float*, const StepArgs&, ReactionArgs, hipStream_t);
+                                         ReactionFold, hipStream_t);
+template void launch_node_update<float>(const NodeArrays&, const float*, const StepArgs&, ReactionArgs, ReactionFold,
+                                        hipStream_t);
 
 void launch_uq_closure(const NodeArrays& na, const UqClosure& uc, const StepArgs& sa, hipStream_t s)
 {

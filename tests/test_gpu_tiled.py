@@ -195,14 +195,19 @@ def test_tiled_matches_slots_over_a_run(hip_device, monkeypatch):
     assert eu <= 1e-7 and eR <= 1e-6
 
 
-def test_reaction_history_independent_of_call_chunking(hip_device):
-    """A step's reaction total is summed inside the next step's element
-    launch, or by a separate launch after a call's last step; both use the
-    same fixed order, so chunking the run differently changes no bit."""
+@pytest.mark.parametrize("mode", ["tiles", "slots", "fp64"])
+def test_reaction_history_independent_of_call_chunking(hip_device, monkeypatch, mode):
+    """A step's reaction total is summed inside the next step's first launch
+    (tiled element kernel, or the node kernel of the slot and FP64 paths), or
+    by a separate launch after a call's last step; both use the same fixed
+    order, so chunking the run differently changes no bit."""
+    if mode != "fp64":
+        monkeypatch.setenv("NSM_MIXED_ASSEMBLY", mode)
     prob = N.CubeProblem(n=12, materials=[N.problems.STEEL, N.problems.ALUMINIUM], num_steps=40)
     hist = []
     for chunks in ([40], [1, 7, 32], [13, 1, 1, 25]):
-        m = prob.make_model("mixed")
+        m = prob.make_model("fp64" if mode == "fp64" else "mixed")
+        assert m.AssemblyInfo()["tiled"] == (mode == "tiles")
         integ = prob.make_integrator(m)
         for c in chunks:
             integ.Advance(c)
