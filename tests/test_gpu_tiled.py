@@ -216,3 +216,26 @@ def test_reaction_history_independent_of_call_chunking(hip_device, monkeypatch, 
     for R, u in hist[1:]:
         assert np.array_equal(R, hist[0][0]) and np.array_equal(u, hist[0][1])
     assert np.all(hist[0][0][1:] != 0.0)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_nodes_without_elements(hip_device, precision):
+    """Nodes no element touches have zero lumped mass and get the reference's
+    a = (1/0)(0 + 0) = NaN (src/integrators/explicit_time_integrator.cc:252-255),
+    exactly where the oracle has it; every other node is unaffected."""
+    n = 6
+    X, conn = N.jittered_cube(n, 13, 0.1)
+    Xo = np.concatenate([X, [[0.3, 0.4, 0.5], [0.7, 0.2, 0.5], [0.5, 0.5, 0.5]]])
+    mats = [N.NeohookeanMaterial.from_young(7800, 200e9, 0.3), N.NeohookeanMaterial.from_young(2700, 70e9, 0.33)]
+    steps = 20
+    m, run = run_both(Xo, conn, [0, n ** 3 // 2, n ** 3], mats, steps, precision)
+    assert m.AssemblyInfo()["tiled"] == (precision == "mixed")
+    u, v = m.GetVectorNodeData("displacement"), m.GetVectorNodeData("velocity")
+    nan_ref = np.isnan(run.v).any(axis=1)
+    assert nan_ref[-3:].all() and not nan_ref[:-3].any()
+    assert np.array_equal(np.isnan(v).any(axis=1), nan_ref) and np.array_equal(np.isnan(u).any(axis=1), nan_ref)
+    if precision == "fp64":
+        assert np.array_equal(u, run.u, equal_nan=True) and np.array_equal(v, run.v, equal_nan=True)
+    else:
+        assert rel_err(u[:-3], run.u[:-3]) <= 1e-6 and rel_err(v[:-3], run.v[:-3]) <= 1e-5
+    m.close()
