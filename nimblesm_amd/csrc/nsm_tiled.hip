@@ -54,6 +54,25 @@ __device__ __forceinline__ T tile_table(const T* p, int t)
   return ((const __attribute__((address_space(4))) T*)p)[t];
 }
 
+// F / sigma rows go out as buffer stores: the resource covers one tile's
+// block (base and size uniform), the row offset is a scalar operand and the
+// lane's element offset one vector register shared by all 120 stores, so
+// the stores need no per-store address arithmetic.  A store past the block
+// is dropped by the range check.  0x00020000: gfx9 resource word 3 (32-bit
+// data format, which untyped buffer accesses need).
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t tile_rsrc(float* p, int bytes)
+{
+  return __builtin_amdgcn_make_buffer_rsrc(p, 0, bytes, 0x00020000);
+}
+
+constexpr int kNonTemporal = 2;  // cache-policy operand: nt
+
+template <int kAux>
+__device__ __forceinline__ void store_row(float v, __amdgpu_buffer_rsrc_t r, int lane_off, int row_off)
+{
+  __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, v), r, lane_off, row_off, kAux);
+}
+
 // Node ids of the (at most two) tile-node slots a thread stages, with the
 // slot's partial destination (kInterior for a tile-interior node).  A thread
 // past the tile's last slot gets that last slot's node (so every load stays
@@ -248,20 +267,18 @@ __global__ __launch_bounds__(kThreads) void k_element_tiled(const BlockTable<flo
       // its n elements at [72 e0, 72 (e0 + n)) and [48 e0, 48 (e0 + n)), so
       // each tile writes two contiguous blocks; non-temporal stores (the
       // step never reads them back) keep node data and partials in cache
-      float* const Ft    = kStoreIp ? Fip + (int64_t)e0 * 72 : ta.ip_sink + (t & 7) * 120 * ntl;
-      float* const St    = kStoreIp ? sip + (int64_t)e0 * 48 : Ft + 72 * ntl;
-      auto         store = [&](int q, const float* Fq, const float* Sq) {
-        if constexpr (kStoreIp) {
+      float* const Ft   = kStoreIp ? Fip + (int64_t)e0 * 72 : ta.ip_sink + (t & 7) * 120 * ntl;
+      float* const St   = kStoreIp ? sip + (int64_t)e0 * 48 : Ft + 72 * ntl;
+      const auto   rF   = tile_rsrc(Ft, 72 * 4 * ntl);
+      const auto   rS   = tile_rsrc(St, 48 * 4 * ntl);
+      const int    loff = 4 * lcl;
+      // the sink is rewritten every 8 tiles: keep it in L2
+      constexpr int kAux  = kStoreIp ? kNonTemporal : 0;
+      auto          store = [&](int q, const float* Fq, const float* Sq) {
 #pragma unroll
-          for (int c = 0; c < 9; c++) __builtin_nontemporal_store(Fq[c], Ft + (c * 8 + q) * ntl + lcl);
+        for (int c = 0; c < 9; c++) store_row<kAux>(Fq[c], rF, loff, 4 * (c * 8 + q) * ntl);
 #pragma unroll
-          for (int c = 0; c < 6; c++) __builtin_nontemporal_store(Sq[c], St + (c * 8 + q) * ntl + lcl);
-        } else {  // the sink is rewritten every 8 tiles: keep it in L2
-#pragma unroll
-          for (int c = 0; c < 9; c++) Ft[(c * 8 + q) * ntl + lcl] = Fq[c];
-#pragma unroll
-          for (int c = 0; c < 6; c++) St[(c * 8 + q) * ntl + lcl] = Sq[c];
-        }
+        for (int c = 0; c < 6; c++) store_row<kAux>(Sq[c], rS, loff, 4 * (c * 8 + q) * ntl);
       };
       if (!mixed::element(Xe, ue, bt.K[b], bt.G[b], store, f)) bad = 1;
     }
