@@ -200,12 +200,14 @@ int nsm_model_get_node_order(nsm_model* model, int64_t* out);
 int nsm_model_get_element_order(nsm_model* model, int64_t* out);
 /* How the model assembles nodal forces: *tiled = 1 for tile-local assembly
  * (mixed precision), with the number of element tiles, of nodes updated
- * inside their tile and of nodes updated from per-tile partial sums;
+ * inside their tile and of nodes updated from per-tile partial sums, and
+ * the number of tiles that sum each node's element forces per corner index
+ * (the others sum them per element colour);
  * *tiled = 0 for element force slots + CSR node gather (all nodes "boundary").
  * No reference counterpart (the reference scatters element forces,
  * src/nimble_kokkos_model_data.cc:1259-1280). */
 int nsm_model_assembly_info(nsm_model* model, int* tiled, int64_t* num_tiles, int64_t* interior_nodes,
-                            int64_t* boundary_nodes);
+                            int64_t* boundary_nodes, int64_t* gather_tiles);
 
 int nsm_model_synchronize(nsm_model* model);
 /* Launch-duration statistics gathered by HIP events around every launch of

@@ -61,7 +61,8 @@ SIGNATURES = {
     "nsm_model_kernel_time": (_I, [_P, _I, C.POINTER(_D), C.POINTER(_I64)]),
     "nsm_model_get_element_order": (_I, [_P, _P]),
     "nsm_model_get_node_order": (_I, [_P, _P]),
-    "nsm_model_assembly_info": (_I, [_P, C.POINTER(_I), C.POINTER(_I64), C.POINTER(_I64), C.POINTER(_I64)]),
+    "nsm_model_assembly_info": (_I, [_P, C.POINTER(_I), C.POINTER(_I64), C.POINTER(_I64), C.POINTER(_I64),
+                                       C.POINTER(_I64)]),
     "nsm_model_info": (_I, [_P, C.POINTER(_I64), C.POINTER(_I64), C.POINTER(_I64), C.POINTER(_I64)]),
     "nsm_jittered_cube": (_I, [_I, C.c_uint64, _D, _P, _P]),
 }

@@ -626,6 +626,7 @@ int setup_tiles(nsm_model* md)
       (st = upload(md, ts.nodes, &ta.nodes)) || (st = upload(md, ts.slot_dst, &ta.slot_dst)) ||
       (st = upload(md, ts.lconn, &ta.lconn)) || (st = upload(md, ts.color, &ta.color)) ||
       (st = upload(md, std::vector<int32_t>(ts.num_colors.begin(), ts.num_colors.end()), &ta.num_colors)) ||
+      (st = upload(md, std::vector<int32_t>(ts.gather.begin(), ts.gather.end()), &ta.gather)) ||
       (st = upload(md, ts.bnd_ptr, &ta.bnd_ptr)))
     return st;
   float* partials = nullptr;
@@ -850,6 +851,10 @@ int nsm_model_create(int64_t num_nodes, const double* reference_coordinates, int
       std::string err;
       md->tiled = nsm::build_tiles(num_nodes, reference_coordinates, conn, off, tile_elems, md->tiles, &err);
       if (!md->tiled) md->tiles = nsm::TileSet();
+      // NSM_TILE_ASSEMBLY=colors: colour-ordered assembly in every tile
+      // (otherwise tiles that allow it gather per corner)
+      const char* ta = std::getenv("NSM_TILE_ASSEMBLY");
+      if (ta && std::strcmp(ta, "colors") == 0) std::fill(md->tiles.gather.begin(), md->tiles.gather.end(), 0);
     }
   }
 
@@ -1432,10 +1437,11 @@ int nsm_model_get_element_order(nsm_model* md, int64_t* out)
 }
 
 int nsm_model_assembly_info(nsm_model* md, int* tiled, int64_t* num_tiles, int64_t* interior_nodes,
-                            int64_t* boundary_nodes)
+                            int64_t* boundary_nodes, int64_t* gather_tiles)
 {
   if (!md) return fail(NSM_ERR_ARG, "null model");
   if (tiled) *tiled = md->tiled ? 1 : 0;
+  if (gather_tiles) *gather_tiles = std::count(md->tiles.gather.begin(), md->tiles.gather.end(), (uint8_t)1);
   if (num_tiles) *num_tiles = md->tiles.num_tiles;
   if (interior_nodes) *interior_nodes = md->tiles.num_interior;
   if (boundary_nodes) *boundary_nodes = md->tiled ? (int64_t)md->tiles.bnd_nodes.size() : md->N;

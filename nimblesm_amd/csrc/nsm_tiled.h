@@ -23,6 +23,7 @@ struct TileArgs {
   const uint16_t* lconn;       // [E][8], internal element order
   const uint8_t*  color;       // [E]
   const int32_t*  num_colors;  // [T]
+  const int32_t*  gather;      // [T] nonzero: per-corner gather assembly (TileSet::gather)
   int             bnd_base;    // device id of boundary node 0 (boundary node b is bnd_base + b)
   const int32_t*  bnd_ptr;     // [B+1]
   float*          partials;    // [partials][3]; node b's are [bnd_ptr[b], bnd_ptr[b+1])

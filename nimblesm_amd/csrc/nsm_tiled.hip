@@ -121,7 +121,25 @@ __device__ __forceinline__ ElemData load_elem(const TileArgs& ta, int t)
   return d;
 }
 
-// Staging of one tile: its nodes' X and u into LDS, zeroed accumulators,
+// LDS of the force assembly.  Colour tiles: one FP64 accumulator per node
+// slot.  Gather tiles: per node slot, the element holding it at each corner
+// index (kNoElement: none), and the elements' FP32 forces component-major
+// (ef[3 j + c][element]: neighbouring elements in neighbouring banks, for
+// the writes and for the gathers of neighbouring slots), so a slot sums its
+// at most 8 contributions in ascending corner order after one barrier.
+// Column kThreads of ef is zero (absent corners read it); staging a gather
+// tile rewrites it, since a colour tile's accumulators overlap it.
+constexpr uint16_t kNoElement = 0xffff;
+template <int kThreads>
+union Assembly {
+  double acc[2 * kThreads][3];
+  struct {
+    uint16_t inv[2 * kThreads][8];
+    float    ef[24][kThreads + 1];
+  } g;
+};
+
+// Staging of one tile: its nodes' X and u into LDS, cleared assembly slots,
 // and the loads of its interior nodes' flags (and v, m when the step updates
 // velocities).  Values of non-interior slots are left unset: they are never
 // read, and no merge of a default forces an early wait.
@@ -133,18 +151,23 @@ struct NodeState {
 template <int kThreads>
 __device__ __forceinline__ void stage(const SlotIds& ids, const double (&xr)[2][3], const double (&ur)[2][3],
                                       const NodeArrays& na, const StepArgs& sa, double (*sX)[3], double (*su)[3],
-                                      double (*acc)[3], NodeState& ns)
+                                      Assembly<kThreads>& as, bool gather, NodeState& ns)
 {
+  if (gather && threadIdx.x < 24) as.g.ef[threadIdx.x][kThreads] = 0.0f;
 #pragma unroll
   for (int k = 0; k < 2; k++) {
     const int ln = (int)threadIdx.x + k * kThreads;
     if (ids.dst[k] != kAbsent) {
 #pragma unroll
       for (int c = 0; c < 3; c++) {
-        sX[ln][c]  = xr[k][c];
-        su[ln][c]  = ur[k][c];
-        acc[ln][c] = 0.0;
+        sX[ln][c] = xr[k][c];
+        su[ln][c] = ur[k][c];
       }
+      if (gather)
+        *reinterpret_cast<uint4*>(as.g.inv[ln]) = make_uint4(~0u, ~0u, ~0u, ~0u);
+      else
+#pragma unroll
+        for (int c = 0; c < 3; c++) as.acc[ln][c] = 0.0;
     }
     if (ids.dst[k] == kInterior) {
       ns.flg[k] = na.flags[ids.gid[k]];
@@ -167,9 +190,11 @@ __device__ __forceinline__ void stage(const SlotIds& ids, const double (&xr)[2][
 // UQ samples, which keep no element data) sends the same stores to a small
 // sink that stays in L2 instead of F and sigma: without them the compiler
 // schedules the element arithmetic with more registers than a wave pair
-// has, and the kernel is no faster for the bytes it saves.
+// has, and the kernel is no faster for the bytes it saves.  The launch bound
+// holds the kernel to two waves per SIMD (at most 256 registers): left free,
+// the scheduler takes more for the gather assembly and halves occupancy.
 template <int kThreads, bool kStoreIp>
-__global__ __launch_bounds__(kThreads) void k_element_tiled(const BlockTable<float> bt,
+__global__ __launch_bounds__(kThreads, 2) void k_element_tiled(const BlockTable<float> bt,
                                                             TileArgs ta, int t_begin, int t_count,
                                                             const double* __restrict__ X, NodeArrays na,
                                                             float* __restrict__ Fip, float* __restrict__ sip,
@@ -179,7 +204,7 @@ __global__ __launch_bounds__(kThreads) void k_element_tiled(const BlockTable<flo
   constexpr int kNodes = 2 * kThreads;
   __shared__ double  sX[kNodes][3];
   __shared__ double  su[kNodes][3];
-  __shared__ double  acc[kNodes][3];
+  __shared__ __attribute__((aligned(16))) Assembly<kThreads> as;
   __shared__ double  red[kThreads / 64];
   __shared__ BcEntry sbc[kMaxBcEntries];
   __shared__ int     bad;
@@ -219,7 +244,7 @@ __global__ __launch_bounds__(kThreads) void k_element_tiled(const BlockTable<flo
   }
   ElemData  cur = load_elem<kThreads>(ta, tb);
   NodeState ns;
-  stage<kThreads>(ids, xr, ur, na, sa, sX, su, acc, ns);
+  stage<kThreads>(ids, xr, ur, na, sa, sX, su, as, tile_table(ta.gather, tb) != 0, ns);
   // every prologue load complete (vmcnt(0); lgkmcnt and expcnt at their
   // maximum, gfx9 encoding): the loop's waits then depend on its body only
   __builtin_amdgcn_s_waitcnt(0x0F70);
@@ -283,30 +308,46 @@ __global__ __launch_bounds__(kThreads) void k_element_tiled(const BlockTable<flo
       if (!mixed::element(Xe, ue, bt.K[b], bt.G[b], store, f)) bad = 1;
     }
 
-    // ---- colour-ordered assembly: the elements of one colour touch distinct
-    // nodes.  An element's 8 nodes are distinct unless it is flagged
-    // kRepeatedNode, so its 24 updates are done as one batch of reads, adds
-    // and writes; a flagged element adds node by node.
-    const int  col      = cur.cinfo == 0xff ? -1 : (cur.cinfo & kColorMask);
-    const bool repeated = cur.cinfo != 0xff && (cur.cinfo & kRepeatedNode) != 0;
-    for (int c = 0; c < ncol; c++) {
-      if (col == c) {
-        if (!repeated) {
-          double a[8][3];
+    const bool gather = tile_table(ta.gather, t) != 0;  // uniform
+    if (gather) {
+      // ---- gather tiles: every element publishes its forces and, for each
+      // corner j, itself as the slot's corner-j element (no other element of
+      // the tile holds that slot at corner j)
+      if (tid < ntl) {
 #pragma unroll
-          for (int j = 0; j < 8; j++)
+        for (int j = 0; j < 8; j++) as.g.inv[lc[j]][j] = (uint16_t)tid;
 #pragma unroll
-            for (int k = 0; k < 3; k++) a[j][k] = acc[lc[j]][k];
+        for (int j = 0; j < 8; j++)
 #pragma unroll
-          for (int j = 0; j < 8; j++)
-#pragma unroll
-            for (int k = 0; k < 3; k++) acc[lc[j]][k] = a[j][k] + (double)f[j][k];
-        } else {
-          for (int j = 0; j < 8; j++)
-            for (int k = 0; k < 3; k++) acc[lc[j]][k] += (double)f[j][k];
-        }
+          for (int c = 0; c < 3; c++) as.g.ef[3 * j + c][tid] = f[j][c];
       }
       __syncthreads();
+    } else {
+      // ---- colour-ordered assembly: the elements of one colour touch
+      // distinct nodes.  An element's 8 nodes are distinct unless it is
+      // flagged kRepeatedNode, so its 24 updates are done as one batch of
+      // reads, adds and writes; a flagged element adds node by node.
+      const int  col      = cur.cinfo == 0xff ? -1 : (cur.cinfo & kColorMask);
+      const bool repeated = cur.cinfo != 0xff && (cur.cinfo & kRepeatedNode) != 0;
+      for (int c = 0; c < ncol; c++) {
+        if (col == c) {
+          if (!repeated) {
+            double a[8][3];
+#pragma unroll
+            for (int j = 0; j < 8; j++)
+#pragma unroll
+              for (int k = 0; k < 3; k++) a[j][k] = as.acc[lc[j]][k];
+#pragma unroll
+            for (int j = 0; j < 8; j++)
+#pragma unroll
+              for (int k = 0; k < 3; k++) as.acc[lc[j]][k] = a[j][k] + (double)f[j][k];
+          } else {
+            for (int j = 0; j < 8; j++)
+              for (int k = 0; k < 3; k++) as.acc[lc[j]][k] += (double)f[j][k];
+          }
+        }
+        __syncthreads();
+      }
     }
 
     // ---- interior nodes: complete step; others: this tile's partial
@@ -315,7 +356,23 @@ __global__ __launch_bounds__(kThreads) void k_element_tiled(const BlockTable<flo
     for (int k = 0; k < 2; k++) {
       const int ln = tid + k * kThreads;
       if (ids.dst[k] == kAbsent) continue;
-      double fn[3] = {acc[ln][0], acc[ln][1], acc[ln][2]};
+      double fn[3];
+      if (gather) {
+        // the slot's contributions in ascending corner order, from 0.0
+        const uint4    iv   = *reinterpret_cast<const uint4*>(as.g.inv[ln]);
+        const unsigned w[4] = {iv.x, iv.y, iv.z, iv.w};
+        fn[0] = fn[1] = fn[2] = 0.0;
+#pragma unroll
+        for (int j = 0; j < 8; j++) {
+          const unsigned e  = (w[j >> 1] >> (16 * (j & 1))) & 0xffffu;
+          const int      ec = e == kNoElement ? kThreads : (int)e;
+#pragma unroll
+          for (int c = 0; c < 3; c++) fn[c] += (double)as.g.ef[3 * j + c][ec];
+        }
+      } else {
+#pragma unroll
+        for (int c = 0; c < 3; c++) fn[c] = as.acc[ln][c];
+      }
       if (ids.dst[k] == kInterior) {
         if (rx.coord >= 0 && (ns.flg[k] & kReactionFlag)) rxs += fn[rx.coord];
         double un[3] = {su[ln][0], su[ln][1], su[ln][2]};
@@ -337,7 +394,7 @@ __global__ __launch_bounds__(kThreads) void k_element_tiled(const BlockTable<flo
     ids      = ids_next;
     ids_next = ids_after;
     cur      = el_next;
-    if (t + G < te) stage<kThreads>(ids, xr, ur, na, sa, sX, su, acc, ns);
+    if (t + G < te) stage<kThreads>(ids, xr, ur, na, sa, sX, su, as, tile_table(ta.gather, t + G) != 0, ns);
   }
   if (tid == 0 && bad) atomicOr(status, 1);
 }

@@ -112,8 +112,10 @@ bool build_tiles(int64_t num_nodes, const double* X, const int* conn, const std:
       std::sort(tnodes.begin(), tnodes.end());
       for (size_t i = 0; i < tnodes.size(); i++) slot_of[(size_t)tnodes[i]] = (int)i;
       std::vector<uint32_t> used(tnodes.size(), 0u);  // colours used at each node slot
+      std::vector<uint8_t>  corners(tnodes.size(), 0);  // corner indices at which a slot occurs
       std::vector<int32_t>  cnt(tnodes.size(), 0);
-      int                   ncol = 0;
+      int                   ncol   = 0;
+      bool                  gather = true;
       for (int64_t p = tstart; p < pos; p++) {
         const int64_t e    = ts.perm[(size_t)p];
         uint32_t      busy = 0u;
@@ -122,6 +124,8 @@ bool build_tiles(int64_t num_nodes, const double* X, const int* conn, const std:
           ts.lconn[(size_t)(8 * p + j)] = (uint16_t)s;
           busy |= used[(size_t)s];
           cnt[(size_t)s]++;
+          gather = gather && !(corners[(size_t)s] >> j & 1);
+          corners[(size_t)s] |= (uint8_t)(1u << j);
         }
         int c = 0;
         while (c < kMaxColors && (busy >> c & 1u)) c++;
@@ -164,6 +168,7 @@ bool build_tiles(int64_t num_nodes, const double* X, const int* conn, const std:
         slot_of[(size_t)tnodes[i]] = -1;
       }
       ts.num_colors.push_back((uint8_t)ncol);
+      ts.gather.push_back((uint8_t)gather);
       ts.elem_off.push_back((int32_t)pos);
       ts.node_off.push_back((int32_t)ts.nodes.size());
     }

@@ -6,10 +6,12 @@
 // "interior" to it: that tile sums its force completely and performs its
 // central-difference update in the element kernel.  Every other node gets one
 // partial sum per tile that touches it; the boundary kernel adds those
-// partials in ascending tile order.  Inside a tile, element forces are summed
-// into the node accumulators one element colour at a time (no two elements of
-// a colour share a node), so the summation order is fixed and runs are
-// reproducible.
+// partials in ascending tile order.  Inside a tile, a node's force is the sum
+// of its element contributions in ascending corner index when no two elements
+// of the tile hold that node at the same corner index (every tile of a
+// structured mesh; TileSet::gather), otherwise the sum over element colours
+// in ascending colour (no two elements of a colour share a node).  Either
+// way the summation order is fixed and runs are reproducible.
 #pragma once
 
 #include <stdint.h>
@@ -36,6 +38,8 @@ struct TileSet {
   std::vector<uint16_t> lconn;          // [E][8] local node slot of each element node (internal order)
   std::vector<uint8_t>  color;          // [E] colour | kRepeatedNode (internal order)
   std::vector<uint8_t>  num_colors;     // [T]
+  std::vector<uint8_t>  gather;         // [T] 1: no two elements of the tile share a node at the same
+                                        // corner index, so each node's forces can be gathered per corner
   std::vector<int32_t>  bnd_nodes;      // boundary nodes, ascending
   std::vector<int32_t>  bnd_ptr;        // [B+1]
   std::vector<int32_t>  bnd_ent;        // tile-node slot of each partial, ascending tile

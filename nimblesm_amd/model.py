@@ -231,9 +231,10 @@ class ModelData:
         return out
 
     def AssemblyInfo(self):
-        t, a, b, c = C.c_int(), C.c_int64(), C.c_int64(), C.c_int64()
-        check(lib().nsm_model_assembly_info(self._h, C.byref(t), C.byref(a), C.byref(b), C.byref(c)))
-        return dict(tiled=bool(t.value), num_tiles=a.value, interior_nodes=b.value, boundary_nodes=c.value)
+        t, a, b, c, g = C.c_int(), C.c_int64(), C.c_int64(), C.c_int64(), C.c_int64()
+        check(lib().nsm_model_assembly_info(self._h, C.byref(t), C.byref(a), C.byref(b), C.byref(c), C.byref(g)))
+        return dict(tiled=bool(t.value), num_tiles=a.value, interior_nodes=b.value, boundary_nodes=c.value,
+                    gather_tiles=g.value)
 
     # -- reaction-force history
     def SetReactionNodes(self, nodes, coordinate=2):

@@ -195,6 +195,54 @@ def test_tiled_matches_slots_over_a_run(hip_device, monkeypatch):
     assert eu <= 1e-7 and eR <= 1e-6
 
 
+def test_gather_and_colour_assembly_agree(hip_device, monkeypatch):
+    """Tiles of a lattice sum each node's forces per corner index; with
+    NSM_TILE_ASSEMBLY=colors the same tiles sum them per element colour.  The
+    FP64 sum of at most 8 FP32 forces is almost always exact in either order,
+    so the two runs agree to 1e-12 (usually bit for bit)."""
+    prob = N.CubeProblem(n=16, materials=[N.problems.STEEL, N.problems.ALUMINIUM], num_steps=200)
+    res = {}
+    for mode in ("gather", "colors"):
+        if mode == "colors":
+            monkeypatch.setenv("NSM_TILE_ASSEMBLY", "colors")
+        m = prob.make_model("mixed")
+        info = m.AssemblyInfo()
+        assert info["tiled"] and info["gather_tiles"] == (info["num_tiles"] if mode == "gather" else 0)
+        prob.make_integrator(m).Integrate()
+        res[mode] = [m.GetVectorNodeData(f) for f in ("displacement", "velocity", "internal_force")]
+        res[mode].append(m.GetReactionHistory(0, prob.num_steps))
+        m.close()
+    same = all(np.array_equal(a, b) for a, b in zip(res["gather"], res["colors"]))
+    errs = [rel_err(a, b) for a, b in zip(res["gather"], res["colors"])]
+    print("gather vs colours after 200 steps: u %.1e v %.1e f %.1e R %.1e" % tuple(errs), "bitwise" if same else "")
+    assert max(errs) <= 1e-12
+
+
+def test_mixed_gather_and_colour_tiles_against_oracle(hip_device):
+    """Elements with their corners relabelled (a quarter turn about their z
+    axis) hold nodes at the same corner index as a neighbour: their tiles use
+    the colour order, the rest gather per corner, in one launch."""
+    n = 12
+    X, conn = N.jittered_cube(n, 13, 0.1)
+    rot = np.arange(300, 900)
+    conn[rot] = conn[rot][:, [1, 2, 3, 0, 5, 6, 7, 4]]
+    mats = [N.NeohookeanMaterial.from_young(7800, 200e9, 0.3), N.NeohookeanMaterial.from_young(2700, 70e9, 0.33)]
+    steps = 60
+    m, run = run_both(X, conn, [0, n ** 3 // 2, n ** 3], mats, steps)
+    info = m.AssemblyInfo()
+    assert info["tiled"] and 0 < info["gather_tiles"] < info["num_tiles"]
+    eu = rel_err(m.GetVectorNodeData("displacement"), run.u)
+    ev = rel_err(m.GetVectorNodeData("velocity"), run.v)
+    ef = rel_err(m.GetVectorNodeData("internal_force"), run.f)
+    eR = rel_err(m.GetReactionHistory(0, steps), run.reaction)
+    _, Fo, so = run.model.internal_force(run.u, want_ip=True)
+    eF = rel_err(m.GetIntegrationPointData("deformation_gradient"), Fo)
+    es = rel_err(m.GetIntegrationPointData("stress"), so)
+    print(f"{info['gather_tiles']}/{info['num_tiles']} gather tiles vs oracle: u {eu:.2e} v {ev:.2e} f {ef:.2e} "
+          f"R {eR:.2e} F {eF:.2e} sigma {es:.2e}")
+    assert eu <= 1e-6 and ev <= 1e-5 and ef <= 1e-5 and eR <= 1e-5 and eF <= 1e-6 and es <= 1e-5
+
+
 @pytest.mark.parametrize("mode", ["tiles", "slots", "fp64"])
 def test_reaction_history_independent_of_call_chunking(hip_device, monkeypatch, mode):
     """A step's reaction total is summed inside the next step's first launch

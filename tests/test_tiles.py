@@ -2,9 +2,10 @@
 (nimblesm_amd/csrc/nsm_tiles.cpp), compiled with g++ from its source and
 checked on the CPU: every element lands in exactly one tile of its block,
 the local connectivity maps back to the mesh under the builder's node
-renumbering, colours separate elements that share a node, and interior nodes
-+ ordered boundary partials reproduce the element-force scatter sum of every
-node exactly once."""
+renumbering, colours separate elements that share a node, tiles flagged for
+the per-corner gather hold each node at most once per corner index, and
+interior nodes + ordered boundary partials reproduce the element-force
+scatter sum of every node exactly once."""
 import ctypes as C
 import os
 import subprocess
@@ -31,7 +32,7 @@ def tiles_lib(tmp_path_factory):
     lib.tiles_build.restype = C.c_void_p
     lib.tiles_build.argtypes = [C.c_int64, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_int]
     lib.tiles_sizes.argtypes = [C.c_void_p, C.c_void_p]
-    lib.tiles_copy.argtypes = [C.c_void_p] + [C.c_void_p] * 14
+    lib.tiles_copy.argtypes = [C.c_void_p] + [C.c_void_p] * 15
     lib.tiles_free.argtypes = [C.c_void_p]
     return lib
 
@@ -51,11 +52,11 @@ def build(lib, X, conn, blk_off, tile_elems=256):
         nodes=np.zeros(S, np.int32), node_info=np.zeros(S, np.uint8), lconn=np.zeros((E, 8), np.uint16),
         color=np.zeros(E, np.uint8), num_colors=np.zeros(T, np.uint8), bnd_nodes=np.zeros(B, np.int32),
         bnd_ptr=np.zeros(B + 1, np.int32), bnd_ent=np.zeros(BE, np.int32), slot_dst=np.zeros(S, np.int32),
-        node_new=np.zeros(len(X), np.int32), int_off=np.zeros(T + 1, np.int32),
+        node_new=np.zeros(len(X), np.int32), int_off=np.zeros(T + 1, np.int32), gather=np.zeros(T, np.uint8),
     )
     lib.tiles_copy(h, *(t[k].ctypes.data for k in ("perm", "elem_off", "node_off", "nodes", "node_info", "lconn",
                                                    "color", "num_colors", "bnd_nodes", "bnd_ptr", "bnd_ent",
-                                                   "slot_dst", "node_new", "int_off")))
+                                                   "slot_dst", "node_new", "int_off", "gather")))
     lib.tiles_free(h)
     t["num_interior"] = int(sz[4])
     t["tile_elems"] = tile_elems
@@ -102,6 +103,9 @@ def check_invariants(t, N, conn_mesh, blk_off):
             members = lc[col == c]
             flat = np.concatenate([np.unique(r) for r in members]) if len(members) else np.zeros(0)
             assert len(flat) == len(np.unique(flat)), "two elements of one colour share a node"
+        # gather flag <=> no slot occurs twice at one corner index
+        unique_corners = all(len(np.unique(lc[:, j])) == len(lc) for j in range(8))
+        assert bool(t["gather"][ti]) == unique_corners
         for s in range(no[ti], no[ti + 1]):
             if t["node_info"][s] & 1:
                 assert int(t["nodes"][s]) not in interior_of
@@ -122,20 +126,26 @@ def check_invariants(t, N, conn_mesh, blk_off):
 
 
 def tiled_assembly(t, N, felem_internal):
-    """Sum element forces the way the kernels do: per tile in colour order,
-    interior nodes complete, the others through partials stored at slot_dst
-    (by node, ascending tile order).  felem_internal is [E][8][3] in internal
-    (tile) element order."""
+    """Sum element forces the way the kernels do: per tile, in ascending
+    corner index for gather tiles and in colour order otherwise; interior
+    nodes complete, the others through partials stored at slot_dst (by node,
+    ascending tile order).  felem_internal is [E][8][3] in internal (tile)
+    element order."""
     f = np.full((N, 3), np.nan)
     eo, no = t["elem_off"], t["node_off"]
     partials = np.full((len(t["bnd_ent"]), 3), np.nan)
     for ti in range(len(eo) - 1):
         acc = np.zeros((no[ti + 1] - no[ti], 3))
         el = np.arange(eo[ti], eo[ti + 1])
-        for c in range(int(t["num_colors"][ti])):
-            for e in el[(t["color"][el] & 0x1F) == c]:
-                for j in range(8):
+        if t["gather"][ti]:
+            for j in range(8):
+                for e in el:  # at most one element holds a slot at corner j
                     acc[t["lconn"][e, j]] += felem_internal[e, j]
+        else:
+            for c in range(int(t["num_colors"][ti])):
+                for e in el[(t["color"][el] & 0x1F) == c]:
+                    for j in range(8):
+                        acc[t["lconn"][e, j]] += felem_internal[e, j]
         for ls in range(len(acc)):
             s = no[ti] + ls
             if t["node_info"][s] & 1:
@@ -170,6 +180,7 @@ def test_structured_cube_tiles_are_boxes(tiles_lib, tile_elems, nodes_per_tile):
     assert len(t["elem_off"]) - 1 == n ** 3 // tile_elems
     assert np.all(np.diff(t["node_off"]) == nodes_per_tile)
     assert np.all(t["num_colors"] == 8)
+    assert np.all(t["gather"] == 1)  # a lattice holds every node once per corner index
     assert len(interior) > 0.3 * len(X)
 
 
@@ -202,9 +213,14 @@ def test_unstructured_permuted_mesh_with_orphans_and_collapsed_elements(tiles_li
     for e in (0, 17, 400):
         conn[e, 5] = conn[e, 4]
         conn[e, 6] = conn[e, 7]
+    # relabel the corners of some elements (a quarter turn about their z axis):
+    # their tiles hold nodes twice at one corner index and fall back to colours
+    rot = np.arange(500, 700)
+    conn[rot] = conn[rot][:, [1, 2, 3, 0, 5, 6, 7, 4]]
     blk_off = np.array([0, 1, 333, 333, len(conn)])  # a one-element block and an empty block
     t = build(tiles_lib, Xn, conn, blk_off, 128)
     check_invariants(t, N, conn, blk_off)
+    assert 0 < int(t["gather"].sum()) < len(t["gather"])  # both assembly orders occur
     felem = rng.normal(size=(len(conn), 8, 3))
     ref = np.zeros((N, 3))
     np.add.at(ref, t["node_new"][conn].reshape(-1), felem.reshape(-1, 3))

@@ -39,7 +39,8 @@ void tiles_sizes(const void* h, int64_t out[5])
 
 void tiles_copy(const void* h, int64_t* perm, int32_t* elem_off, int32_t* node_off, int32_t* nodes,
                 uint8_t* node_info, uint16_t* lconn, uint8_t* color, uint8_t* num_colors, int32_t* bnd_nodes,
-                int32_t* bnd_ptr, int32_t* bnd_ent, int32_t* slot_dst, int32_t* node_new, int32_t* int_off)
+                int32_t* bnd_ptr, int32_t* bnd_ent, int32_t* slot_dst, int32_t* node_new, int32_t* int_off,
+                uint8_t* gather)
 {
   const auto* ts = static_cast<const nsm::TileSet*>(h);
   put(perm, ts->perm);
@@ -56,6 +57,7 @@ void tiles_copy(const void* h, int64_t* perm, int32_t* elem_off, int32_t* node_o
   put(slot_dst, ts->slot_dst);
   put(node_new, ts->node_new);
   put(int_off, ts->int_off);
+  put(gather, ts->gather);
 }
 
 void tiles_free(void* h) { delete static_cast<nsm::TileSet*>(h); }
