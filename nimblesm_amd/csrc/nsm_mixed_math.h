@@ -188,20 +188,30 @@ struct ForceMoments {
   float A[9], Be[9], Bf[9], C[9];
 };
 
-template <int SX, int SY, int SZ>
-__host__ __device__ __forceinline__ void accumulate(ForceMoments& fm, const float M[9])
+// Two Gauss points that differ only in the sign of xi (Mm at -g, Mp at +g),
+// with the signs SY, SZ they share: their sum P and xi-weighted difference D
+// enter the moments of the x-row (e = y, f = z: constant over the pair) and
+// of the y- and z-rows (e = x: D; f = z or y: constant).
+template <int SY, int SZ>
+__host__ __device__ __forceinline__ void accumulate_pair(ForceMoments& fm, const float Mm[9], const float Mp[9])
 {
-  // m = x: e = y, f = z;  m = y: e = x, f = z;  m = z: e = x, f = y
-  constexpr int se[3] = {SY, SX, SX};
-  constexpr int sf[3] = {SZ, SZ, SY};
-  for (int m = 0; m < 3; m++)
-    for (int c = 0; c < 3; c++) {
-      const float v = M[3 * m + c];
-      fm.A[3 * m + c] += v;
-      fm.Be[3 * m + c] += (float)se[m] * v;
-      fm.Bf[3 * m + c] += (float)sf[m] * v;
-      fm.C[3 * m + c] += (float)(se[m] * sf[m]) * v;
-    }
+  float P[9], D[9];
+  for (int i = 0; i < 9; i++) P[i] = Mm[i] + Mp[i];
+  for (int i = 3; i < 9; i++) D[i] = Mp[i] - Mm[i];
+  for (int c = 0; c < 3; c++) {
+    fm.A[c] += P[c];
+    fm.Be[c] += (float)SY * P[c];
+    fm.Bf[c] += (float)SZ * P[c];
+    fm.C[c] += (float)(SY * SZ) * P[c];
+    fm.A[3 + c] += P[3 + c];
+    fm.Be[3 + c] += D[3 + c];
+    fm.Bf[3 + c] += (float)SZ * P[3 + c];
+    fm.C[3 + c] += (float)SZ * D[3 + c];
+    fm.A[6 + c] += P[6 + c];
+    fm.Be[6 + c] += D[6 + c];
+    fm.Bf[6 + c] += (float)SY * P[6 + c];
+    fm.C[6 + c] += (float)SY * D[6 + c];
+  }
 }
 
 // Node forces f[n][c] (with the reference's minus sign) from the moments.
@@ -247,23 +257,27 @@ __host__ __device__ __forceinline__ bool element(const double X[8][3], const dou
   ForceMoments fm;
   for (int i = 0; i < 9; i++) fm.A[i] = fm.Be[i] = fm.Bf[i] = fm.C[i] = 0.0f;
   bool ok = true;
-#define NSM_MIXED_POINT(Q, SX, SY, SZ)                \
+#define NSM_MIXED_POINT(Q, SX, SY, SZ, M)             \
   {                                                   \
-    float Jx[9], Ju[9], Fq[9], Sq[6], Mq[9];          \
+    float Jx[9], Ju[9], Fq[9], Sq[6];                 \
     jacobian<SX, SY, SZ>(mx, Jx);                     \
     jacobian<SX, SY, SZ>(mu, Ju);                     \
-    ok = point(Jx, Ju, K, G, Fq, Sq, Mq) && ok;       \
+    ok = point(Jx, Ju, K, G, Fq, Sq, M) && ok;        \
     store(Q, Fq, Sq);                                 \
-    accumulate<SX, SY, SZ>(fm, Mq);                   \
   }
-  NSM_MIXED_POINT(0, -1, -1, -1)
-  NSM_MIXED_POINT(1, 1, -1, -1)
-  NSM_MIXED_POINT(2, 1, 1, -1)
-  NSM_MIXED_POINT(3, -1, 1, -1)
-  NSM_MIXED_POINT(4, -1, -1, 1)
-  NSM_MIXED_POINT(5, 1, -1, 1)
-  NSM_MIXED_POINT(6, 1, 1, 1)
-  NSM_MIXED_POINT(7, -1, 1, 1)
+  // points in pairs (xi = -g, +g) with the same eta, zeta signs
+#define NSM_MIXED_PAIR(QM, QP, SY, SZ)                \
+  {                                                   \
+    float Mm[9], Mp[9];                               \
+    NSM_MIXED_POINT(QM, -1, SY, SZ, Mm)               \
+    NSM_MIXED_POINT(QP, 1, SY, SZ, Mp)                \
+    accumulate_pair<SY, SZ>(fm, Mm, Mp);              \
+  }
+  NSM_MIXED_PAIR(0, 1, -1, -1)
+  NSM_MIXED_PAIR(3, 2, 1, -1)
+  NSM_MIXED_PAIR(4, 5, -1, 1)
+  NSM_MIXED_PAIR(7, 6, 1, 1)
+#undef NSM_MIXED_PAIR
 #undef NSM_MIXED_POINT
   forces_from_moments(fm, f);
   return ok;
