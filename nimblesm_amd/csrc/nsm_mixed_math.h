@@ -89,6 +89,25 @@ __host__ __device__ __forceinline__ void jacobian(const Modes& m, float J[9])
   }
 }
 
+// Jacobians at the two Gauss points (-g, SY g, SZ g) and (+g, SY g, SZ g):
+// the d/dxi column is common, the others are a common part plus or minus
+// their xi-dependent part.
+template <int SY, int SZ>
+__host__ __device__ __forceinline__ void jacobian_pair(const Modes& m, float Jm[9], float Jp[9])
+{
+  for (int i = 0; i < 3; i++) {
+    const float j0 = m.c[0][i] + (float)SY * m.c[3][i] + (float)SZ * m.c[5][i] + (float)(SY * SZ) * m.c[6][i];
+    const float a1 = m.c[1][i] + (float)SZ * m.c[4][i], b1 = m.c[3][i] + (float)SZ * m.c[6][i];
+    const float a2 = m.c[2][i] + (float)SY * m.c[4][i], b2 = m.c[5][i] + (float)SY * m.c[6][i];
+    Jm[3 * i + 0] = j0;
+    Jp[3 * i + 0] = j0;
+    Jm[3 * i + 1] = a1 - b1;
+    Jp[3 * i + 1] = a1 + b1;
+    Jm[3 * i + 2] = a2 - b2;
+    Jp[3 * i + 2] = a2 + b2;
+  }
+}
+
 __host__ __device__ __forceinline__ void adjugate(const float m[9], float adj[9])
 {
   adj[0] = m[4] * m[8] - m[5] * m[7];
@@ -257,20 +276,20 @@ __host__ __device__ __forceinline__ bool element(const double X[8][3], const dou
   ForceMoments fm;
   for (int i = 0; i < 9; i++) fm.A[i] = fm.Be[i] = fm.Bf[i] = fm.C[i] = 0.0f;
   bool ok = true;
-#define NSM_MIXED_POINT(Q, SX, SY, SZ, M)             \
+#define NSM_MIXED_POINT(Q, Jx, Ju, M)                 \
   {                                                   \
-    float Jx[9], Ju[9], Fq[9], Sq[6];                 \
-    jacobian<SX, SY, SZ>(mx, Jx);                     \
-    jacobian<SX, SY, SZ>(mu, Ju);                     \
+    float Fq[9], Sq[6];                               \
     ok = point(Jx, Ju, K, G, Fq, Sq, M) && ok;        \
     store(Q, Fq, Sq);                                 \
   }
   // points in pairs (xi = -g, +g) with the same eta, zeta signs
 #define NSM_MIXED_PAIR(QM, QP, SY, SZ)                \
   {                                                   \
-    float Mm[9], Mp[9];                               \
-    NSM_MIXED_POINT(QM, -1, SY, SZ, Mm)               \
-    NSM_MIXED_POINT(QP, 1, SY, SZ, Mp)                \
+    float Jxm[9], Jxp[9], Jum[9], Jup[9], Mm[9], Mp[9]; \
+    jacobian_pair<SY, SZ>(mx, Jxm, Jxp);              \
+    jacobian_pair<SY, SZ>(mu, Jum, Jup);              \
+    NSM_MIXED_POINT(QM, Jxm, Jum, Mm)                 \
+    NSM_MIXED_POINT(QP, Jxp, Jup, Mp)                 \
     accumulate_pair<SY, SZ>(fm, Mm, Mp);              \
   }
   NSM_MIXED_PAIR(0, 1, -1, -1)
