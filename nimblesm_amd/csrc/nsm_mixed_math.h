@@ -78,20 +78,13 @@ __host__ __device__ __forceinline__ void make_modes(const double v[8][3], Modes&
   }
 }
 
-// Jacobian J[i][d] = dx_i / dxi_d at the Gauss point with signs (sx, sy, sz)
-template <int SX, int SY, int SZ>
-__host__ __device__ __forceinline__ void jacobian(const Modes& m, float J[9])
-{
-  for (int i = 0; i < 3; i++) {
-    J[3 * i + 0] = m.c[0][i] + (float)SY * m.c[3][i] + (float)SZ * m.c[5][i] + (float)(SY * SZ) * m.c[6][i];
-    J[3 * i + 1] = m.c[1][i] + (float)SX * m.c[3][i] + (float)SZ * m.c[4][i] + (float)(SX * SZ) * m.c[6][i];
-    J[3 * i + 2] = m.c[2][i] + (float)SX * m.c[5][i] + (float)SY * m.c[4][i] + (float)(SX * SY) * m.c[6][i];
-  }
-}
-
-// Jacobians at the two Gauss points (-g, SY g, SZ g) and (+g, SY g, SZ g):
-// the d/dxi column is common, the others are a common part plus or minus
-// their xi-dependent part.
+// Jacobian J[i][d] = dx_i / dxi_d at the Gauss point with signs (sx, sy, sz):
+//   J[i][0] = c0 + sy c3 + sz c5 + sy sz c6,
+//   J[i][1] = c1 + sx c3 + sz c4 + sx sz c6,
+//   J[i][2] = c2 + sx c5 + sy c4 + sx sy c6   (c = m.c[.][i]),
+// evaluated for the two points (-g, SY g, SZ g) and (+g, SY g, SZ g): the
+// d/dxi column is common, the others are a common part plus or minus their
+// xi-dependent part.
 template <int SY, int SZ>
 __host__ __device__ __forceinline__ void jacobian_pair(const Modes& m, float Jm[9], float Jp[9])
 {
