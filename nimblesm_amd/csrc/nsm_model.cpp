@@ -840,13 +840,14 @@ int nsm_model_create(int64_t num_nodes, const double* reference_coordinates, int
   if (precision == NSM_MIXED && E > 0) {
     const char* mode = std::getenv("NSM_MIXED_ASSEMBLY");
     if (!(mode && std::strcmp(mode, "slots") == 0)) {
-      // NSM_TILE_ELEMS=128 or 64 selects smaller tiles (two waves / one wave
-      // per workgroup), 512 larger ones (eight waves, one workgroup per CU)
+      // default 512-element tiles (eight waves, one workgroup per CU);
+      // NSM_TILE_ELEMS=256, 128 or 64 selects smaller ones (four, two, one
+      // waves per workgroup)
       const char* te         = std::getenv("NSM_TILE_ELEMS");
       const int   tile_elems = !te                          ? nsm::kTileElems
+                               : std::strcmp(te, "256") == 0 ? 256
                                : std::strcmp(te, "128") == 0 ? 128
                                : std::strcmp(te, "64") == 0  ? 64
-                               : std::strcmp(te, "512") == 0 ? 512
                                                              : nsm::kTileElems;
       std::string err;
       md->tiled = nsm::build_tiles(num_nodes, reference_coordinates, conn, off, tile_elems, md->tiles, &err);

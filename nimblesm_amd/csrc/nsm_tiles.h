@@ -1,7 +1,7 @@
 // nsm_tiles.h -- element tiles with tile-local force assembly (mixed path).
 //
 // Elements of each block are put in Morton order of their centroids and cut
-// into tiles of at most tile_elems elements (256 by default; 64, 128, 512) touching
+// into tiles of at most tile_elems elements (512 by default; 64, 128, 256) touching
 // at most 2*tile_elems distinct nodes.  A node whose incident elements all lie in one tile is
 // "interior" to it: that tile sums its force completely and performs its
 // central-difference update in the element kernel.  Every other node gets one
@@ -21,7 +21,7 @@
 
 namespace nsm {
 
-constexpr int     kTileElems    = 256;  // default elements per tile (one thread each)
+constexpr int     kTileElems    = 512;  // default elements per tile (one thread each)
 constexpr int     kMaxColors    = 32;
 constexpr uint8_t kColorMask    = 0x1f;  // colour bits of TileSet::color
 constexpr uint8_t kRepeatedNode = 0x80;  // element lists one node more than once

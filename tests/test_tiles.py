@@ -37,7 +37,7 @@ def tiles_lib(tmp_path_factory):
     return lib
 
 
-def build(lib, X, conn, blk_off, tile_elems=256):
+def build(lib, X, conn, blk_off, tile_elems=512):
     X = np.ascontiguousarray(X, np.float64)
     conn = np.ascontiguousarray(conn, np.int32)
     off = np.ascontiguousarray(blk_off, np.int64)
@@ -168,15 +168,16 @@ def cube(n, seed=5):
     return X, conn
 
 
-@pytest.mark.parametrize("tile_elems,nodes_per_tile", [(256, 9 * 9 * 5), (128, 9 * 5 * 5), (64, 5 * 5 * 5)])
+@pytest.mark.parametrize("tile_elems,nodes_per_tile",
+                         [(512, 9 * 9 * 9), (256, 9 * 9 * 5), (128, 9 * 5 * 5), (64, 5 * 5 * 5)])
 def test_structured_cube_tiles_are_boxes(tiles_lib, tile_elems, nodes_per_tile):
     n = 16
     X, conn = cube(n)
     blk_off = np.array([0, n ** 3 // 2, n ** 3])
     t = build(tiles_lib, X, conn, blk_off, tile_elems)
     interior = check_invariants(t, len(X), conn, blk_off)
-    # Morton tiles of 256 (128) elements on the jittered lattice are 8x8x4
-    # (8x4x4) boxes of 9x9x5 (9x5x5) nodes
+    # Morton tiles of 512 (256, 128, 64) elements on the jittered lattice are
+    # 8x8x8 (8x8x4, 8x4x4, 4x4x4) boxes of 9x9x9 (9x9x5, 9x5x5, 5x5x5) nodes
     assert len(t["elem_off"]) - 1 == n ** 3 // tile_elems
     assert np.all(np.diff(t["node_off"]) == nodes_per_tile)
     assert np.all(t["num_colors"] == 8)
