@@ -3,11 +3,13 @@
 // k_element_tiled: one workgroup per element tile (nsm_tiles.h), one thread
 // per element.  The tile's nodes (X, u in FP64) are staged in LDS; each
 // thread runs the mixed-precision element arithmetic (nsm_mixed_math.h),
-// writes F and sigma, and its 24 node forces are added into FP64 LDS
-// accumulators colour by colour (fixed order).  Tile-interior nodes then take
-// their whole central-difference update here; every other node gets this
-// tile's partial sum (FP64 in LDS, stored in FP32).  k_boundary_update adds the partials of the remaining
-// nodes in ascending tile order and updates them.  Each kernel also leaves one
+// writes F and sigma, and its 24 node forces are summed per node in a fixed
+// order: gathered per corner index after one barrier where no node repeats
+// a corner index in the tile, else added into FP64 LDS accumulators colour
+// by colour.  Tile-interior nodes then take their whole central-difference
+// update here; every other node gets this tile's partial sum (FP64, stored
+// in FP32).  k_boundary_update adds the partials of the remaining nodes in
+// ascending tile order and updates them.  Each kernel also leaves one
 // reaction partial per workgroup.
 //
 // The loads a tile needs are issued before its element arithmetic (one tile
