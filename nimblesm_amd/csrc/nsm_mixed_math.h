@@ -56,8 +56,12 @@ __host__ __device__ __forceinline__ void walsh7(const double v[8], double w[7])
   w[6] = dd1 - dd0;  // W_xyz
 }
 
-// Modal coefficients scaled for evaluation at Gauss points: c[0..2] = W_x,y,z / 8,
-// c[3..5] = g W_xy,yz,xz / 8, c[6] = g^2 W_xyz / 8.
+// Modal coefficients scaled for evaluation at Gauss points: c[0..2] = W_x,y,z,
+// c[3..5] = g W_xy,yz,xz, c[6] = g^2 W_xyz.  That is 8 times the Jacobian's
+// coefficients (the 1/8 of the shape functions is left out): a power of
+// two, so every quantity formed from them is exactly 8^k times its value
+// and rounds the same -- H, F, J and sigma are unchanged, the force moments
+// are 64 times theirs and forces_from_moments divides by 64.
 struct Modes {
   float c[7][3];  // [mode][component]
 };
@@ -68,13 +72,13 @@ __host__ __device__ __forceinline__ void make_modes(const double v[8][3], Modes&
     const double col[8] = {v[0][i], v[1][i], v[2][i], v[3][i], v[4][i], v[5][i], v[6][i], v[7][i]};
     double       w[7];
     walsh7(col, w);
-    m.c[0][i] = (float)(w[0] * 0.125);
-    m.c[1][i] = (float)(w[1] * 0.125);
-    m.c[2][i] = (float)(w[2] * 0.125);
-    m.c[3][i] = (float)(w[3] * (0.125 * 0.577350269189626));
-    m.c[4][i] = (float)(w[4] * (0.125 * 0.577350269189626));
-    m.c[5][i] = (float)(w[5] * (0.125 * 0.577350269189626));
-    m.c[6][i] = (float)(w[6] * (0.125 * 0.577350269189626 * 0.577350269189626));
+    m.c[0][i] = (float)w[0];
+    m.c[1][i] = (float)w[1];
+    m.c[2][i] = (float)w[2];
+    m.c[3][i] = (float)(w[3] * 0.577350269189626);
+    m.c[4][i] = (float)(w[4] * 0.577350269189626);
+    m.c[5][i] = (float)(w[5] * 0.577350269189626);
+    m.c[6][i] = (float)(w[6] * (0.577350269189626 * 0.577350269189626));
   }
 }
 
@@ -136,7 +140,7 @@ __host__ __device__ __forceinline__ float rcp(float x)
 
 // One Gauss point: F (K_F order), sigma (K_S order) and M = adj(a) sigma
 // (row-major), from the reference Jacobian Jx and displacement Jacobian Ju.
-// Returns false when det(Jx) or det(a) is not positive.
+// Returns false when det(Jx) or det(a) = J det(Jx) is not positive.
 __host__ __device__ __forceinline__ bool point(const float Jx[9], const float Ju[9], float K, float G, float F[9],
                                                float S[6], float M[9])
 {
@@ -186,11 +190,11 @@ __host__ __device__ __forceinline__ bool point(const float Jx[9], const float Ju
   float a[9], adja[9];
   for (int i = 0; i < 9; i++) a[i] = Jx[i] + Ju[i];
   adjugate(a, adja);
-  const float deta = a[0] * adja[0] + a[1] * adja[3] + a[2] * adja[6];
   const float Sm[9] = {S[0], S[3], S[5], S[3], S[1], S[4], S[5], S[4], S[2]};
   for (int m = 0; m < 3; m++)
     for (int c = 0; c < 3; c++) M[3 * m + c] = adja[3 * m] * Sm[c] + adja[3 * m + 1] * Sm[3 + c] + adja[3 * m + 2] * Sm[6 + c];
-  return detb > 0.0f && deta > 0.0f;
+  // det(a) = J det(Jx): with det(Jx) > 0, a is inverted exactly when J <= 0
+  return detb > 0.0f && J > 0.0f;
 }
 
 // Modal moments of M over the Gauss points (signs only; g factors applied at
@@ -229,7 +233,8 @@ __host__ __device__ __forceinline__ void accumulate_pair(ForceMoments& fm, const
 // Node forces f[n][c] (with the reference's minus sign) from the moments.
 __host__ __device__ __forceinline__ void forces_from_moments(const ForceMoments& fm, float f[8][3])
 {
-  const float s1 = -0.125f * kG, s2 = -0.125f * kG2, s0 = -0.125f;
+  // -1/8 of the shape functions, and 1/64 for the moments' scale (Modes)
+  const float s0 = -0.125f / 64.0f, s1 = s0 * kG, s2 = s0 * kG2;
   for (int c = 0; c < 3; c++) {
     // mode coefficients of the force field
     const float cx   = s0 * fm.A[0 + c];
