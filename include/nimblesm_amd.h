@@ -183,7 +183,8 @@ int nsm_model_set_field(nsm_model* model, int field, const double* in);
  * nsm_model_get_element_order) are element-fastest [comp][ipt][E] in FP64 and
  * untiled mixed models; tiled mixed models store them per element tile: the
  * tile holding device elements [e0, e0 + n) owns [8 nc e0, 8 nc (e0 + n)),
- * value (c, q) of its element e0 + l at 8 nc e0 + (8 c + q) n + l. */
+ * value (c, q) of its element e0 + l at 8 nc e0 + (4 c + q/2) 2n + 2l + q%2
+ * (rows of 2n values: the element's points q and q+1, q even, side by side). */
 int nsm_model_field_device_pointer(nsm_model* model, int field, void** ptr, int* elem_bytes);
 /* Node order of the device node arrays: out[i] (i < N) is the mesh node
  * stored at device position i.  FP64 models keep the mesh numbering; tiled

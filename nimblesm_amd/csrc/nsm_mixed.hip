@@ -51,7 +51,7 @@ __global__ __launch_bounds__(kThreads) NSM_MIXED_WAVES_ATTR void k_element_mixed
   }
   // F and sigma are not read again by the step: non-temporal stores (none
   // when Fip is null)
-  auto store = [&](int q, const float* Fq, const float* Sq) {
+  auto store_point = [&](int q, const float* Fq, const float* Sq) {
     if (!Fip) return;
 #pragma unroll
     for (int c = 0; c < 9; c++) __builtin_nontemporal_store(Fq[c], Fip + (int64_t)(c * 8 + q) * E_total + e);
@@ -59,6 +59,10 @@ __global__ __launch_bounds__(kThreads) NSM_MIXED_WAVES_ATTR void k_element_mixed
     for (int c = 0; c < 6; c++) __builtin_nontemporal_store(Sq[c], sip + (int64_t)(c * 8 + q) * E_total + e);
   };
   float      f[8][3];
+  auto store = [&](int qm, int qp, const float* Fm, const float* Sm, const float* Fp, const float* Sp) {
+    store_point(qm, Fm, Sm);
+    store_point(qp, Fp, Sp);
+  };
   const bool ok = mixed::element(Xe, ue, K, G, store, f);
 #pragma unroll
   for (int n = 0; n < 8; n++)

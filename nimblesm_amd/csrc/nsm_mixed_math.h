@@ -263,7 +263,9 @@ __host__ __device__ __forceinline__ void forces_from_moments(const ForceMoments&
 }
 
 // Whole element.  Gauss point q has signs (sx, sy, sz) of node q
-// (src/nimble_element.cc:60-90).  F [8][9], S [8][6], f [8][3].
+// (src/nimble_element.cc:60-90).  F and sigma of each pair of points
+// (0,1), (3,2), (4,5), (7,6) go to store(q_minus, q_plus, F-, S-, F+, S+)
+// (F in K_F order, sigma in K_S order); f [8][3].
 template <typename Store>
 __host__ __device__ __forceinline__ bool element(const double X[8][3], const double u[8][3], float K, float G,
                                                  Store&& store, float f[8][3])
@@ -274,28 +276,23 @@ __host__ __device__ __forceinline__ bool element(const double X[8][3], const dou
   ForceMoments fm;
   for (int i = 0; i < 9; i++) fm.A[i] = fm.Be[i] = fm.Bf[i] = fm.C[i] = 0.0f;
   bool ok = true;
-#define NSM_MIXED_POINT(Q, Jx, Ju, M)                 \
-  {                                                   \
-    float Fq[9], Sq[6];                               \
-    ok = point(Jx, Ju, K, G, Fq, Sq, M) && ok;        \
-    store(Q, Fq, Sq);                                 \
-  }
-  // points in pairs (xi = -g, +g) with the same eta, zeta signs
-#define NSM_MIXED_PAIR(QM, QP, SY, SZ)                \
-  {                                                   \
-    float Jxm[9], Jxp[9], Jum[9], Jup[9], Mm[9], Mp[9]; \
-    jacobian_pair<SY, SZ>(mx, Jxm, Jxp);              \
-    jacobian_pair<SY, SZ>(mu, Jum, Jup);              \
-    NSM_MIXED_POINT(QM, Jxm, Jum, Mm)                 \
-    NSM_MIXED_POINT(QP, Jxp, Jup, Mp)                 \
-    accumulate_pair<SY, SZ>(fm, Mm, Mp);              \
+  // points in pairs (xi = -g, +g) with the same eta, zeta signs; the pair's
+  // F and sigma go to store together
+#define NSM_MIXED_PAIR(QM, QP, SY, SZ)                                               \
+  {                                                                                  \
+    float Jxm[9], Jxp[9], Jum[9], Jup[9], Mm[9], Mp[9], Fm[9], Sm[6], Fp[9], Sp[6]; \
+    jacobian_pair<SY, SZ>(mx, Jxm, Jxp);                                             \
+    jacobian_pair<SY, SZ>(mu, Jum, Jup);                                             \
+    ok = point(Jxm, Jum, K, G, Fm, Sm, Mm) && ok;                                    \
+    ok = point(Jxp, Jup, K, G, Fp, Sp, Mp) && ok;                                    \
+    store(QM, QP, Fm, Sm, Fp, Sp);                                                   \
+    accumulate_pair<SY, SZ>(fm, Mm, Mp);                                             \
   }
   NSM_MIXED_PAIR(0, 1, -1, -1)
   NSM_MIXED_PAIR(3, 2, 1, -1)
   NSM_MIXED_PAIR(4, 5, -1, 1)
   NSM_MIXED_PAIR(7, 6, 1, 1)
 #undef NSM_MIXED_PAIR
-#undef NSM_MIXED_POINT
   forces_from_moments(fm, f);
   return ok;
 }

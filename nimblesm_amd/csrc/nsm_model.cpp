@@ -228,7 +228,8 @@ inline int64_t mesh_node(const nsm_model* md, int64_t i) { return md->tiled ? md
 // Calls f(device index, device element, component, point) for every value of
 // an integration-point field with nc components: element-fastest rows
 // [c][q][E] in untiled models; in tiled ones each tile t (device elements
-// [e0, e0 + n)) owns [8 nc e0, 8 nc (e0 + n)) with rows (8 c + q) of length n.
+// [e0, e0 + n)) owns [8 nc e0, 8 nc (e0 + n)) with rows (4 c + q/2) of length
+// 2n, element l's points q and q+1 (q even) side by side.
 template <typename Fn>
 void for_each_ip(const nsm_model* md, int nc, Fn&& fn)
 {
@@ -243,7 +244,7 @@ void for_each_ip(const nsm_model* md, int nc, Fn&& fn)
     const int64_t e0 = eo[(size_t)t], n = eo[(size_t)t + 1] - e0, base = 8 * nc * e0;
     for (int c = 0; c < nc; c++)
       for (int q = 0; q < 8; q++)
-        for (int64_t l = 0; l < n; l++) fn(base + (c * 8 + q) * n + l, e0 + l, c, q);
+        for (int64_t l = 0; l < n; l++) fn(base + (c * 4 + q / 2) * 2 * n + 2 * l + (q & 1), e0 + l, c, q);
   }
 }
 

@@ -39,9 +39,9 @@ unsigned      boundary_grid(int num_bnd);
 // partials of the others.  Reaction partials go to rx.partials[tile].
 // Workgroup 0 first writes the folded reaction total of a previous step.
 // F and sigma are tile-blocked: component c, point q of local element l of
-// tile t (elements [e0, e0 + n)) at Fip[72 e0 + (8 c + q) n + l] and
-// sip[48 e0 + (8 c + q) n + l]; with Fip = sip = nullptr they go to
-// ta.ip_sink (overwritten, cache resident) instead.
+// tile t (elements [e0, e0 + n)) at Fip[72 e0 + (4 c + q/2) 2n + 2l + q%2]
+// and sip[48 e0 + (4 c + q/2) 2n + 2l + q%2]; with Fip = sip = nullptr they
+// go to ta.ip_sink (overwritten, cache resident) instead.
 void launch_element_tiled(const BlockTable<float>& bt, const TileArgs& ta, int t_begin, int t_count,
                           const double* X, const NodeArrays& na, float* Fip, float* sip, const StepArgs& sa,
                           ReactionArgs rx, ReactionFold fold, int* status, hipStream_t s);

@@ -58,7 +58,7 @@ __device__ __forceinline__ T tile_table(const T* p, int t)
 
 // F / sigma rows go out as buffer stores: the resource covers one tile's
 // block (base and size uniform), the row offset is a scalar operand and the
-// lane's element offset one vector register shared by all 120 stores, so
+// lane's element offset one vector register shared by all 60 stores, so
 // the stores need no per-store address arithmetic.  A store past the block
 // is dropped by the range check.  0x00020000: gfx9 resource word 3 (32-bit
 // data format, which untyped buffer accesses need).
@@ -69,10 +69,15 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t tile_rsrc(float* p, int bytes)
 
 constexpr int kNonTemporal = 2;  // cache-policy operand: nt
 
+// Values of points 2p and 2p+1 of one component, side by side: one 8-byte
+// store per lane.
 template <int kAux>
-__device__ __forceinline__ void store_row(float v, __amdgpu_buffer_rsrc_t r, int lane_off, int row_off)
+__device__ __forceinline__ void store_pair(float even, float odd, __amdgpu_buffer_rsrc_t r, int lane_off,
+                                           int row_off)
 {
-  __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, v), r, lane_off, row_off, kAux);
+  typedef unsigned v2u __attribute__((ext_vector_type(2)));
+  const v2u d = {__builtin_bit_cast(uint32_t, even), __builtin_bit_cast(uint32_t, odd)};
+  __builtin_amdgcn_raw_buffer_store_b64(d, r, lane_off, row_off, kAux);
 }
 
 // Node ids of the (at most two) tile-node slots a thread stages, with the
@@ -290,22 +295,29 @@ __global__ __launch_bounds__(kThreads, 2) void k_element_tiled(const BlockTable<
           ue[j][c] = su[lc[j]][c];
         }
       const int b = uniform(block_of(bt, (int64_t)e0));  // tiles never straddle blocks
-      // tile-blocked F and sigma: tile t owns rows (component, point) of
-      // its n elements at [72 e0, 72 (e0 + n)) and [48 e0, 48 (e0 + n)), so
-      // each tile writes two contiguous blocks; non-temporal stores (the
-      // step never reads them back) keep node data and partials in cache
+      // tile-blocked F and sigma: tile t owns [72 e0, 72 (e0 + n)) and
+      // [48 e0, 48 (e0 + n)), in rows (component c, point pair p) of 2n
+      // values, element l's points 2p and 2p+1 side by side, so each tile
+      // writes two contiguous blocks with 8-byte stores; non-temporal stores
+      // (the step never reads them back) keep node data and partials in cache
       float* const Ft   = kStoreIp ? Fip + (int64_t)e0 * 72 : ta.ip_sink + (t & 7) * 120 * ntl;
       float* const St   = kStoreIp ? sip + (int64_t)e0 * 48 : Ft + 72 * ntl;
       const auto   rF   = tile_rsrc(Ft, 72 * 4 * ntl);
       const auto   rS   = tile_rsrc(St, 48 * 4 * ntl);
-      const int    loff = 4 * lcl;
+      const int    loff = 8 * lcl;
       // the sink is rewritten every 8 tiles: keep it in L2
       constexpr int kAux  = kStoreIp ? kNonTemporal : 0;
-      auto          store = [&](int q, const float* Fq, const float* Sq) {
+      auto store = [&](int qm, int qp, const float* Fm, const float* Sm, const float* Fp, const float* Sp) {
+        const bool   m_even = qm < qp;  // qm, qp: constants after inlining
+        const int    p      = (m_even ? qm : qp) / 2;
+        const float* Fe     = m_even ? Fm : Fp;
+        const float* Fo     = m_even ? Fp : Fm;
+        const float* Se     = m_even ? Sm : Sp;
+        const float* So     = m_even ? Sp : Sm;
 #pragma unroll
-        for (int c = 0; c < 9; c++) store_row<kAux>(Fq[c], rF, loff, 4 * (c * 8 + q) * ntl);
+        for (int c = 0; c < 9; c++) store_pair<kAux>(Fe[c], Fo[c], rF, loff, 8 * (c * 4 + p) * ntl);
 #pragma unroll
-        for (int c = 0; c < 6; c++) store_row<kAux>(Sq[c], rS, loff, 4 * (c * 8 + q) * ntl);
+        for (int c = 0; c < 6; c++) store_pair<kAux>(Se[c], So[c], rS, loff, 8 * (c * 4 + p) * ntl);
       };
       if (!mixed::element(Xe, ue, bt.K[b], bt.G[b], store, f)) bad = 1;
     }
