@@ -157,7 +157,7 @@ struct NodeState {
 
 template <int kThreads>
 __device__ __forceinline__ void stage(const SlotIds& ids, const double (&xr)[2][3], const double (&ur)[2][3],
-                                      const NodeArrays& na, const StepArgs& sa, double (*sX)[3], double (*su)[3],
+                                      const NodeArrays& na, const StepArgs& sa, double (*sxu)[6],
                                       Assembly<kThreads>& as, bool gather, NodeState& ns)
 {
   if (gather && threadIdx.x < 24) as.g.ef[threadIdx.x][kThreads] = 0.0f;
@@ -167,8 +167,8 @@ __device__ __forceinline__ void stage(const SlotIds& ids, const double (&xr)[2][
     if (ids.dst[k] != kAbsent) {
 #pragma unroll
       for (int c = 0; c < 3; c++) {
-        sX[ln][c] = xr[k][c];
-        su[ln][c] = ur[k][c];
+        sxu[ln][c]     = xr[k][c];
+        sxu[ln][3 + c] = ur[k][c];
       }
       if (gather)
         *reinterpret_cast<uint4*>(as.g.inv[ln]) = make_uint4(~0u, ~0u, ~0u, ~0u);
@@ -209,8 +209,9 @@ __global__ __launch_bounds__(kThreads, 2) void k_element_tiled(const BlockTable<
                                                             int* __restrict__ status)
 {
   constexpr int kNodes = 2 * kThreads;
-  __shared__ double  sX[kNodes][3];
-  __shared__ double  su[kNodes][3];
+  // X and u of a node slot side by side (48 bytes): an element reads each
+  // of its nodes with three 16-byte LDS reads
+  __shared__ __attribute__((aligned(16))) double sxu[kNodes][6];
   __shared__ __attribute__((aligned(16))) Assembly<kThreads> as;
   __shared__ double  red[kThreads / 64];
   __shared__ BcEntry sbc[kMaxBcEntries];
@@ -251,7 +252,7 @@ __global__ __launch_bounds__(kThreads, 2) void k_element_tiled(const BlockTable<
   }
   ElemData  cur = load_elem<kThreads>(ta, tb);
   NodeState ns;
-  stage<kThreads>(ids, xr, ur, na, sa, sX, su, as, tile_table(ta.gather, tb) != 0, ns);
+  stage<kThreads>(ids, xr, ur, na, sa, sxu, as, tile_table(ta.gather, tb) != 0, ns);
   // every prologue load complete (vmcnt(0); lgkmcnt and expcnt at their
   // maximum, gfx9 encoding): the loop's waits then depend on its body only
   __builtin_amdgcn_s_waitcnt(0x0F70);
@@ -291,8 +292,8 @@ __global__ __launch_bounds__(kThreads, 2) void k_element_tiled(const BlockTable<
       for (int j = 0; j < 8; j++)
 #pragma unroll
         for (int c = 0; c < 3; c++) {
-          Xe[j][c] = sX[lc[j]][c];
-          ue[j][c] = su[lc[j]][c];
+          Xe[j][c] = sxu[lc[j]][c];
+          ue[j][c] = sxu[lc[j]][3 + c];
         }
       const int b = uniform(block_of(bt, (int64_t)e0));  // tiles never straddle blocks
       // tile-blocked F and sigma: tile t owns [72 e0, 72 (e0 + n)) and
@@ -389,7 +390,7 @@ __global__ __launch_bounds__(kThreads, 2) void k_element_tiled(const BlockTable<
       }
       if (ids.dst[k] == kInterior) {
         if (rx.coord >= 0 && (ns.flg[k] & kReactionFlag)) rxs += fn[rx.coord];
-        double un[3] = {su[ln][0], su[ln][1], su[ln][2]};
+        double un[3] = {sxu[ln][3], sxu[ln][4], sxu[ln][5]};
         node_update(na, sbc, ids.gid[k], ns.flg[k], fn, un, ns.v[k], ns.m[k], sa);
       } else {
         float* const pp = ta.partials + 3 * (int64_t)ids.dst[k];
@@ -408,7 +409,7 @@ __global__ __launch_bounds__(kThreads, 2) void k_element_tiled(const BlockTable<
     ids      = ids_next;
     ids_next = ids_after;
     cur      = el_next;
-    if (t + G < te) stage<kThreads>(ids, xr, ur, na, sa, sX, su, as, tile_table(ta.gather, t + G) != 0, ns);
+    if (t + G < te) stage<kThreads>(ids, xr, ur, na, sa, sxu, as, tile_table(ta.gather, t + G) != 0, ns);
   }
   if (tid == 0 && bad) atomicOr(status, 1);
 }
