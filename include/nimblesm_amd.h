@@ -108,6 +108,21 @@ int nsm_hex_compute_characteristic_length(int64_t num_elem, const double* x_gath
 int nsm_neohookean_get_stress(int64_t num_points, double bulk_modulus, double shear_modulus, const double* F_np1,
                               double* sigma_np1, void* stream);
 
+/* The per-point NeohookeanMaterial::GetStress over the STEP_NP1 views of one
+ * block where they lie, as a block material interface applies it
+ * (compute_block_stress, src/nimble_kokkos_block_material_interface.cc:65-119;
+ * per point src/nimble_material.h:269-286): num_elem x num_ipt points, F
+ * value (e, q, c) at F_np1[e*F_elem_stride + q*F_ipt_stride + c*F_comp_stride],
+ * sigma value at the same form with the sigma strides (device pointers).
+ * Kokkos LayoutLeft [E][8][9] views have strides (1, E, 8E), LayoutRight
+ * ones (72, 9, 1); a model's FP64 arrays (nsm_model_field_device_pointer)
+ * are [comp][ipt][E]: block elements [e0, e0+n) at base + e0 with strides
+ * (1, E, 8E). */
+int nsm_neohookean_get_stress_strided(int64_t num_elem, int num_ipt, double bulk_modulus, double shear_modulus,
+                                      const double* F_np1, int64_t F_elem_stride, int64_t F_ipt_stride,
+                                      int64_t F_comp_stride, double* sigma_np1, int64_t sigma_elem_stride,
+                                      int64_t sigma_ipt_stride, int64_t sigma_comp_stride, void* stream);
+
 /* ---------------------------------------------------------------------------
  * Model-data seam: replaces nimble::ModelDataBase / nimble_kokkos::ModelData
  * (src/nimble_model_data_base.h:87-345, src/nimble_kokkos_model_data.cc) and
@@ -147,6 +162,22 @@ int nsm_model_get_critical_time_step(nsm_model* model, double* dt);
  * f_int, F_np1 and sigma_np1 from the model's current displacement. */
 int nsm_model_compute_internal_force(nsm_model* model, double time_previous, double time_current,
                                      int is_output_step);
+
+/* ComputeInternalForce split around a block material interface, as the
+ * reference's Kokkos ModelData runs it when a BlockMaterialInterfaceFactory
+ * is installed (src/nimble_kokkos_model_data.cc:1196-1257, the interface's
+ * ComputeStress() between them, src/nimble_block_material_interface_base.h:76-84):
+ * nsm_model_compute_deformation_gradients writes F_np1 from the current
+ * displacement; the caller's stress computation then fills sigma_np1
+ * (nsm_model_field_device_pointer; e.g. nsm_neohookean_get_stress_strided on
+ * the model's stream, nsm_model_get_stream); nsm_model_compute_internal_force_from_stress
+ * forms f_int from it.  FP64 models without UQ samples (NSM_ERR_STATE
+ * otherwise).  With the reference's own Neo-Hookean stress the result is
+ * bit-identical to nsm_model_compute_internal_force. */
+int nsm_model_compute_deformation_gradients(nsm_model* model);
+int nsm_model_compute_internal_force_from_stress(nsm_model* model);
+/* The HIP stream (hipStream_t) the model's work is ordered on. */
+int nsm_model_get_stream(nsm_model* model, void** stream);
 
 /* ModelData::UpdateStates (src/nimble_kokkos_model_data.cc:963-1021).  The
  * stateless Neo-Hookean model never reads STEP_N, so this is a no-op kept for

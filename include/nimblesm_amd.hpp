@@ -73,6 +73,15 @@ class NeohookeanMaterial {
   {
     check(nsm_neohookean_get_stress(num_points, bulk_modulus_, shear_modulus_, F_np1, sigma_np1, stream));
   }
+  // Device pointers at (element, point, component) strides: the STEP_NP1
+  // views of one block where they lie.
+  void GetStressStrided(int64_t num_elem, int num_ipt, const double* F_np1, const int64_t F_strides[3],
+                        double* sigma_np1, const int64_t sigma_strides[3], void* stream = nullptr) const
+  {
+    check(nsm_neohookean_get_stress_strided(num_elem, num_ipt, bulk_modulus_, shear_modulus_, F_np1, F_strides[0],
+                                            F_strides[1], F_strides[2], sigma_np1, sigma_strides[0],
+                                            sigma_strides[1], sigma_strides[2], stream));
+  }
 
  private:
   double density_, bulk_modulus_, shear_modulus_;
@@ -134,6 +143,23 @@ class ModelData {
     check(nsm_model_compute_internal_force(h_, t_prev, t_cur, is_output_step ? 1 : 0));
   }
   void UpdateStates() { check(nsm_model_update_states(h_)); }
+  // ComputeInternalForce split around a stress plug-in (FP64 models)
+  void ComputeDeformationGradients() { check(nsm_model_compute_deformation_gradients(h_)); }
+  void ComputeInternalForceFromStress() { check(nsm_model_compute_internal_force_from_stress(h_)); }
+  void* Stream()
+  {
+    void* s = nullptr;
+    check(nsm_model_get_stream(h_, &s));
+    return s;
+  }
+  // device address of a field (see nsm_model_field_device_pointer)
+  void* DevicePointer(nsm_field f, int* elem_bytes = nullptr)
+  {
+    void* p = nullptr;
+    check(nsm_model_field_device_pointer(h_, f, &p, elem_bytes));
+    return p;
+  }
+  void SetScalarNodeData(nsm_field f, const std::vector<double>& v) { check(nsm_model_set_field(h_, f, v.data())); }
   std::vector<double> GetVectorNodeData(nsm_field f)
   {
     std::vector<double> out((size_t)(3 * num_nodes_));
@@ -214,7 +240,7 @@ class ExplicitTimeIntegrator {
     model_.ComputeLumpedMass();
     critical_time_step_ = model_.GetCriticalTimeStep();
     model_.ApplyInitialConditions();
-    model_.ApplyKinematicConditions(0.0, 0.0);
+    model_.ApplyKinematicConditions(time_current_, time_current_);
   }
   void Advance(int64_t nsteps)
   {

@@ -37,6 +37,7 @@ SIGNATURES = {
     "nsm_hex_compute_lumped_mass": (_I, [_I64, _D, _P, _P, _P]),
     "nsm_hex_compute_characteristic_length": (_I, [_I64, _P, _P, _P]),
     "nsm_neohookean_get_stress": (_I, [_I64, _D, _D, _P, _P, _P]),
+    "nsm_neohookean_get_stress_strided": (_I, [_I64, _I, _D, _D, _P, _I64, _I64, _I64, _P, _I64, _I64, _I64, _P]),
     "nsm_model_create": (_I, [_I64, _P, _I, _P, _P, _P, _P, _P, _P, _I, C.POINTER(_P)]),
     "nsm_model_destroy": (_I, [_P]),
     "nsm_model_add_boundary_condition": (_I, [_P, _I, _I, _D, _I64, _P]),
@@ -46,6 +47,9 @@ SIGNATURES = {
     "nsm_model_get_critical_time_step": (_I, [_P, C.POINTER(_D)]),
     "nsm_model_compute_internal_force": (_I, [_P, _D, _D, _I]),
     "nsm_model_update_states": (_I, [_P]),
+    "nsm_model_compute_deformation_gradients": (_I, [_P]),
+    "nsm_model_compute_internal_force_from_stress": (_I, [_P]),
+    "nsm_model_get_stream": (_I, [_P, C.POINTER(_P)]),
     "nsm_model_set_reaction_nodes": (_I, [_P, _I64, _P, _I]),
     "nsm_model_get_reaction_history": (_I, [_P, _I64, _I64, _P]),
     "nsm_model_sample_reaction_history": (_I, [_P, _I64, _I, _P]),

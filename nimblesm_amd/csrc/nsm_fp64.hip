@@ -171,6 +171,12 @@ struct Lds {
   };
 };
 
+// kMode: kElementFull (F, stress and forces), kElementDefGrad (F only, the
+// first kernel of src/nimble_kokkos_model_data.cc:1196-1227) or
+// kElementForcesFromStress (forces from the stress stored in sip, the
+// kernel after a block material interface's ComputeStress,
+// src/nimble_kokkos_model_data.cc:1229-1257).
+template <int kMode>
 __global__ __launch_bounds__(kThreads) void k_element_fp64(const BlockTable<double> bt, int64_t E_total,
                                                            const int* __restrict__ conn,
                                                            const double* __restrict__ X,
@@ -200,21 +206,35 @@ __global__ __launch_bounds__(kThreads) void k_element_fp64(const BlockTable<doub
   double ainv[10], sig[6];
   int    bad = 0;
   if (valid) {
-    double a[9], b[9], bi[9], F[9];
+    double a[9], b[9];
     jacobians(k, lds.nodes.x[el], lds.nodes.u[el], a, b);
-    bad |= invert3x3(b, bi) <= 0.0;
-    def_grad(a, bi, F);
-    // element-fastest [comp][ipt][E] layout (no stores when Fip is null)
-    if (Fip)
+    if (kMode != kElementForcesFromStress) {
+      double bi[9], F[9];
+      bad |= invert3x3(b, bi) <= 0.0;
+      def_grad(a, bi, F);
+      // element-fastest [comp][ipt][E] layout (no stores when Fip is null)
+      if (Fip)
 #pragma unroll
-      for (int i = 0; i < 9; i++) Fip[(int64_t)(i * 8 + k) * E_total + e] = F[i];
-    neohookean_stress(K, G, F, sig);
-    if (sip)
+        for (int i = 0; i < 9; i++) Fip[(int64_t)(i * 8 + k) * E_total + e] = F[i];
+      if (kMode == kElementFull) {
+        neohookean_stress(K, G, F, sig);
+        if (sip)
 #pragma unroll
-      for (int i = 0; i < 6; i++) sip[(int64_t)(i * 8 + k) * E_total + e] = sig[i];
+          for (int i = 0; i < 6; i++) sip[(int64_t)(i * 8 + k) * E_total + e] = sig[i];
+      }
+    } else {
+#pragma unroll
+      for (int i = 0; i < 6; i++) sig[i] = sip[(int64_t)(i * 8 + k) * E_total + e];
+    }
     // the force routine rebuilds a from the same x = X + u in the same order
-    ainv[9] = invert3x3(a, ainv);
-    bad |= ainv[9] <= 0.0;
+    if (kMode != kElementDefGrad) {
+      ainv[9] = invert3x3(a, ainv);
+      bad |= ainv[9] <= 0.0;
+    }
+  }
+  if (kMode == kElementDefGrad) {
+    if (bad) atomicOr(status, 1);
+    return;
   }
   __syncthreads();  // node data no longer needed
   if (valid) {
@@ -302,6 +322,25 @@ __global__ __launch_bounds__(256) void k_seam_stress(int64_t P, double K, double
   for (int i = 0; i < 9; i++) Fp[i] = F[p * 9 + i];
   neohookean_stress(K, G, Fp, s);
   for (int i = 0; i < 6; i++) sig[p * 6 + i] = s[i];
+}
+
+// The same per point over num_elem x num_ipt points at arbitrary strides
+// (point p = e * num_ipt + q), so a block's STEP_NP1 views are used where
+// they lie: Kokkos LayoutLeft [E][8][9], LayoutRight, or a model's
+// [comp][ipt][E] arrays.
+__global__ __launch_bounds__(256) void k_seam_stress_strided(int64_t num_elem, int num_ipt, double K, double G,
+                                                             const double* __restrict__ F, IptStrides fs,
+                                                             double* __restrict__ sig, IptStrides ss)
+{
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= num_elem * num_ipt) return;
+  const int64_t e = p / num_ipt, q = p % num_ipt;
+  double        Fp[9], s[6];
+  const double* Fe = F + e * fs.elem + q * fs.ipt;
+  for (int i = 0; i < 9; i++) Fp[i] = Fe[i * fs.comp];
+  neohookean_stress(K, G, Fp, s);
+  double* Se = sig + e * ss.elem + q * ss.ipt;
+  for (int i = 0; i < 6; i++) Se[i * ss.comp] = s[i];
 }
 
 // Lumped mass: lane q forms det J_q from reference coordinates, lane i then
@@ -447,12 +486,21 @@ inline unsigned grid_of(int64_t n, int64_t per) { return (unsigned)((n + per - 1
 }  // namespace
 
 void launch_element_fp64(const BlockTable<double>& bt, int64_t E_total, const int* conn, const double* X,
-                         const double* u, double* Fip, double* sip, double* slots, int* status, hipStream_t s)
+                         const double* u, double* Fip, double* sip, double* slots, int* status, hipStream_t s,
+                         int mode)
 {
   const int64_t n = bt.off[bt.count] - bt.off[0];
   if (n <= 0) return;
-  hipLaunchKernelGGL(k_element_fp64, dim3(grid_of(n, kEPW)), dim3(kThreads), 0, s, bt, E_total, conn, X, u, Fip, sip,
-                     slots, status);
+  const dim3 grid(grid_of(n, kEPW)), block(kThreads);
+  if (mode == kElementDefGrad)
+    hipLaunchKernelGGL(k_element_fp64<kElementDefGrad>, grid, block, 0, s, bt, E_total, conn, X, u, Fip, sip, slots,
+                       status);
+  else if (mode == kElementForcesFromStress)
+    hipLaunchKernelGGL(k_element_fp64<kElementForcesFromStress>, grid, block, 0, s, bt, E_total, conn, X, u, Fip, sip,
+                       slots, status);
+  else
+    hipLaunchKernelGGL(k_element_fp64<kElementFull>, grid, block, 0, s, bt, E_total, conn, X, u, Fip, sip, slots,
+                       status);
 }
 
 void launch_seam_defgrad(int64_t E, const double* Xg, const double* ug, double* F, int* status, hipStream_t s)
@@ -466,6 +514,15 @@ void launch_seam_forces(int64_t E, const double* Xg, const double* ug, const dou
 {
   if (E <= 0) return;
   hipLaunchKernelGGL(k_seam_forces, dim3(grid_of(E, kEPW)), dim3(kThreads), 0, s, E, Xg, ug, sig, f, status);
+}
+
+void launch_seam_stress_strided(int64_t num_elem, int num_ipt, double K, double G, const double* F, IptStrides fs,
+                                double* sig, IptStrides ss, hipStream_t s)
+{
+  const int64_t P = num_elem * num_ipt;
+  if (P <= 0) return;
+  hipLaunchKernelGGL(k_seam_stress_strided, dim3(grid_of(P, 256)), dim3(256), 0, s, num_elem, num_ipt, K, G, F, fs,
+                     sig, ss);
 }
 
 void launch_seam_stress(int64_t P, double K, double G, const double* F, double* sig, hipStream_t s)

@@ -14,12 +14,18 @@
 
 namespace nsm {
 
-// Per-node resolved kinematic condition.  kind: 0 none, 1 prescribed
-// velocity, 2 prescribed displacement.
+// Per-node resolved kinematic condition, the net effect of the condition
+// list applied in order (src/nimble_boundary_condition_manager.h:136-217).
+// kind: 0 none, 1 prescribed velocity, 2 prescribed displacement, 3
+// prescribed displacement listed after a prescribed velocity: with
+// delta_t <= 0 the displacement condition does nothing, so the velocity
+// condition's value (fallback) is what remains.
+constexpr int32_t kBcVelocity = 1, kBcDisplacement = 2, kBcDisplacementOverVelocity = 3;
 struct BcEntry {
   int32_t kind[3];
   int32_t pad;
   double  value[3];
+  double  fallback[3];
 };
 
 // node_flags: low 7 bits = bc_table index + 1 (0: free), bit 7 = node belongs
@@ -94,8 +100,12 @@ __device__ __forceinline__ int block_of(const BlockTable<Real>& bt, int64_t e)
 }
 
 // ---- element kernels over the elements of a block table ----
+// FP64 element kernel modes: F, stress and forces; F only; forces from the
+// stress already in sip
+constexpr int kElementFull = 0, kElementDefGrad = 1, kElementForcesFromStress = 2;
 void launch_element_fp64(const BlockTable<double>& bt, int64_t E_total, const int* conn, const double* X,
-                         const double* u, double* Fip, double* sip, double* slots, int* status, hipStream_t s);
+                         const double* u, double* Fip, double* sip, double* slots, int* status, hipStream_t s,
+                         int mode = kElementFull);
 void launch_element_mixed(const BlockTable<float>& bt, int64_t E_total, const int* conn, const double* X,
                           const double* u, float* Fip, float* sip, float* slots, int* status, hipStream_t s);
 
@@ -104,6 +114,13 @@ void launch_seam_defgrad(int64_t E, const double* Xg, const double* ug, double* 
 void launch_seam_forces(int64_t E, const double* Xg, const double* ug, const double* sig, double* f, int* status,
                         hipStream_t s);
 void launch_seam_stress(int64_t P, double K, double G, const double* F, double* sig, hipStream_t s);
+// stress of num_elem x num_ipt points: F value (e, q, c) at F[e fe + q fq + c fc],
+// sigma value at sig[e se + q sq + c sc]
+struct IptStrides {
+  int64_t elem, ipt, comp;
+};
+void launch_seam_stress_strided(int64_t num_elem, int num_ipt, double K, double G, const double* F, IptStrides fs,
+                                double* sig, IptStrides ss, hipStream_t s);
 void launch_seam_lumped_mass(int64_t E, double rho, const double* Xg, double* m, int* status, hipStream_t s);
 void launch_seam_char_length(int64_t E, const double* xg, double* out, hipStream_t s);
 

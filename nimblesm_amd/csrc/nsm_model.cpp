@@ -277,9 +277,12 @@ int time_end(nsm_model* md, size_t idx)
   return NSM_OK;
 }
 
-// Resolve the prescribed conditions, in insertion order, to one (kind, value)
-// per node and coordinate: later conditions overwrite earlier ones exactly as
-// the reference's sequential loops do.
+// Resolve the prescribed conditions, in insertion order, to their net effect
+// per node and coordinate, as the reference's sequential loops leave it
+// (src/nimble_boundary_condition_manager.h:136-217): the last condition
+// decides, except that a prescribed displacement does nothing while
+// delta_t <= 0, where the last prescribed velocity before it still holds
+// (nsm::kBcDisplacementOverVelocity).
 int rebuild_bcs(nsm_model* md)
 {
   if (!md->bc_dirty) return NSM_OK;
@@ -297,8 +300,19 @@ int rebuild_bcs(nsm_model* md)
         per_node.push_back(z);
         touched.push_back(n);
       }
-      per_node[it->second].kind[b.coord]  = b.kind;
-      per_node[it->second].value[b.coord] = b.value;
+      nsm::BcEntry& e = per_node[it->second];
+      const int     c = b.coord;
+      if (b.kind == NSM_BC_PRESCRIBED_VELOCITY) {
+        e.kind[c]     = nsm::kBcVelocity;
+        e.value[c]    = b.value;
+        e.fallback[c] = 0.0;
+      } else {  // prescribed displacement: over an earlier velocity condition?
+        const bool over_v = e.kind[c] == nsm::kBcVelocity || e.kind[c] == nsm::kBcDisplacementOverVelocity;
+        const double v    = e.kind[c] == nsm::kBcVelocity ? e.value[c] : e.fallback[c];
+        e.kind[c]         = over_v ? nsm::kBcDisplacementOverVelocity : nsm::kBcDisplacement;
+        e.value[c]        = b.value;
+        e.fallback[c]     = over_v ? v : 0.0;
+      }
     }
   }
   // distinct combinations -> table
@@ -332,16 +346,23 @@ int rebuild_bcs(nsm_model* md)
     NSM_HIP(hipMemcpy(md->bc_table, table.data(), table.size() * sizeof(nsm::BcEntry), hipMemcpyHostToDevice));
   // UQ samples take initial and prescribed velocities but not prescribed
   // displacements (src/nimble_boundary_condition_manager.h:184-215 sets the
-  // nominal velocity only)
+  // nominal velocity only), so a sample keeps the last prescribed velocity:
+  // the value of a velocity entry, the fallback of a displacement entry
+  // listed after one, nothing for a displacement entry without one
   free_dev(md->uq_bc_table);
   if (md->uq_samples() > 0) {
     std::vector<nsm::BcEntry> sample_table(table);
     for (auto& e : sample_table)
-      for (int c = 0; c < 3; c++)
-        if (e.kind[c] == NSM_BC_PRESCRIBED_DISPLACEMENT) {
+      for (int c = 0; c < 3; c++) {
+        if (e.kind[c] == nsm::kBcDisplacementOverVelocity) {
+          e.kind[c]  = nsm::kBcVelocity;
+          e.value[c] = e.fallback[c];
+        } else if (e.kind[c] == nsm::kBcDisplacement) {
           e.kind[c]  = 0;
           e.value[c] = 0.0;
         }
+        e.fallback[c] = 0.0;
+      }
     if ((st = dalloc(&md->uq_bc_table, std::max<size_t>(sample_table.size(), 1), &dummy))) return st;
     if (!sample_table.empty())
       NSM_HIP(hipMemcpy(md->uq_bc_table, sample_table.data(), sample_table.size() * sizeof(nsm::BcEntry),
@@ -420,7 +441,7 @@ nsm::BlockTable<float> float_table(const nsm_model* md, int b0, int nb, int samp
 // All blocks in one launch when they fit one block table, else groups of
 // kMaxLaunchBlocks consecutive blocks; displacement and moduli of the nominal
 // trajectory (sample < 0) or of exact UQ sample `sample`.
-int launch_elements(nsm_model* md, int sample)
+int launch_elements(nsm_model* md, int sample, int fp64_mode = nsm::kElementFull)
 {
   const double* u = md->node_arrays(sample).u;
   for (int b0 = 0; b0 < md->num_blocks; b0 += nsm::kMaxLaunchBlocks) {
@@ -437,7 +458,8 @@ int launch_elements(nsm_model* md, int sample)
         bt.G[b] = md->shear(sample, b0 + b);
       }
       nsm::launch_element_fp64(bt, md->E, md->conn, md->X, u, (double*)ip_out(md, md->Fip, sample),
-                               (double*)ip_out(md, md->sip, sample), (double*)md->slots, md->status, md->stream);
+                               (double*)ip_out(md, md->sip, sample), (double*)md->slots, md->status, md->stream,
+                               fp64_mode);
     } else {
       nsm::launch_element_mixed(float_table(md, b0, nb, sample), md->E, md->conn, md->X, u,
                                 (float*)ip_out(md, md->Fip, sample), (float*)ip_out(md, md->sip, sample),
@@ -1109,6 +1131,61 @@ int nsm_model_compute_internal_force(nsm_model* md, double time_previous, double
   int64_t              pending = -1;
   if ((st = launch_step(md, sa, -1, -1, &pending))) return st;
   return check_status(md);
+}
+
+// The split form of ComputeInternalForce that lets a block material
+// interface compute the stress (src/nimble_kokkos_model_data.cc:1196-1257):
+// F_np1 first, then the plug-in's stress into the model's sigma_np1, then
+// the forces from it.  FP64 models; the nominal trajectory only.
+static int check_split(nsm_model* md)
+{
+  if (!md) return fail(NSM_ERR_ARG, "null model");
+  if (md->precision != NSM_FP64)
+    return fail(NSM_ERR_STATE, "the split force evaluation needs an FP64 model (mixed precision evaluates the "
+                               "stress inside its element kernel)");
+  if (md->uq_samples() > 0) return fail(NSM_ERR_STATE, "the split force evaluation does not advance UQ samples");
+  return ensure_device(md);
+}
+
+int nsm_model_compute_deformation_gradients(nsm_model* md)
+{
+  int st = check_split(md);
+  if (st || (st = launch_elements(md, -1, nsm::kElementDefGrad))) return st;
+  return check_status(md);
+}
+
+int nsm_model_compute_internal_force_from_stress(nsm_model* md)
+{
+  int st = check_split(md);
+  if (st || (st = launch_elements(md, -1, nsm::kElementForcesFromStress))) return st;
+  const nsm::StepTimes none{0.0, 0.0, 0.0, 0.0};
+  const nsm::StepArgs  sa{none, none, 0, 0, 0, 0};
+  int64_t              pending = -1;
+  if ((st = launch_node(md, sa, -1, -1, &pending, -1))) return st;
+  return check_status(md);
+}
+
+int nsm_neohookean_get_stress_strided(int64_t num_elem, int num_ipt, double bulk_modulus, double shear_modulus,
+                                      const double* F_np1, int64_t F_elem_stride, int64_t F_ipt_stride,
+                                      int64_t F_comp_stride, double* sigma_np1, int64_t sigma_elem_stride,
+                                      int64_t sigma_ipt_stride, int64_t sigma_comp_stride, void* stream)
+{
+  if (num_elem < 0 || num_ipt < 0) return fail(NSM_ERR_ARG, "negative size");
+  if (num_elem == 0 || num_ipt == 0) return NSM_OK;
+  if (!F_np1 || !sigma_np1) return fail(NSM_ERR_ARG, "null pointer");
+  nsm::launch_seam_stress_strided(num_elem, num_ipt, bulk_modulus, shear_modulus, F_np1,
+                                  nsm::IptStrides{F_elem_stride, F_ipt_stride, F_comp_stride}, sigma_np1,
+                                  nsm::IptStrides{sigma_elem_stride, sigma_ipt_stride, sigma_comp_stride},
+                                  (hipStream_t)stream);
+  NSM_LAUNCHED();
+  return NSM_OK;
+}
+
+int nsm_model_get_stream(nsm_model* md, void** stream)
+{
+  if (!md || !stream) return fail(NSM_ERR_ARG, "null argument");
+  *stream = (void*)md->stream;
+  return NSM_OK;
 }
 
 int nsm_model_update_states(nsm_model* md)

@@ -29,10 +29,12 @@ __device__ __forceinline__ void apply_bc(const BcEntry* table, int code, double 
   const BcEntry& b = table[code - 1];
 #pragma unroll
   for (int c = 0; c < 3; c++) {
-    if (b.kind[c] == 1)
+    if (b.kind[c] == kBcVelocity)
       v[c] = b.value[c];
-    else if (b.kind[c] == 2 && delta_t > 0.0)
+    else if (b.kind[c] >= kBcDisplacement && delta_t > 0.0)
       v[c] = (b.value[c] - u[c]) / delta_t;
+    else if (b.kind[c] == kBcDisplacementOverVelocity)
+      v[c] = b.fallback[c];
   }
 }
 

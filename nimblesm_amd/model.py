@@ -98,6 +98,14 @@ class NeohookeanMaterial:
         check(lib().nsm_neohookean_get_stress(P, self.bulk_modulus, self.shear_modulus,
                                               _ptr(deformation_gradient_np1), _ptr(stress_np1), _stream(stream)))
 
+    def GetStressStrided(self, num_elem, num_ipt, F_ptr, F_strides, sigma_ptr, sigma_strides, stream=None):
+        """The stress of num_elem x num_ipt points stored at (element, point,
+        component) strides, e.g. one block of a model's STEP_NP1 arrays
+        (device addresses)."""
+        check(lib().nsm_neohookean_get_stress_strided(int(num_elem), int(num_ipt), self.bulk_modulus,
+                                                      self.shear_modulus, F_ptr, *map(int, F_strides), sigma_ptr,
+                                                      *map(int, sigma_strides), _stream(stream)))
+
 
 @dataclass
 class Block:
@@ -178,6 +186,20 @@ class ModelData:
 
     def UpdateStates(self):
         check(lib().nsm_model_update_states(self._h))
+
+    # -- the force evaluation split around a stress plug-in
+    #    (src/nimble_kokkos_model_data.cc:1196-1257; FP64 models)
+    def ComputeDeformationGradients(self):
+        check(lib().nsm_model_compute_deformation_gradients(self._h))
+
+    def ComputeInternalForceFromStress(self):
+        check(lib().nsm_model_compute_internal_force_from_stress(self._h))
+
+    def Stream(self) -> int:
+        """The model's HIP stream (hipStream_t as an integer)."""
+        p = C.c_void_p()
+        check(lib().nsm_model_get_stream(self._h, C.byref(p)))
+        return p.value
 
     def ExplicitSteps(self, step_begin, step_end, num_load_steps, output_frequency, dt_user, time_current):
         t = C.c_double(time_current)
@@ -337,11 +359,15 @@ class ExplicitTimeIntegrator:
         return self
 
     def Integrate(self, callback: Optional[Callable[[int, float], None]] = None):
-        if callback is None:
+        """Every remaining step; ``callback`` after each output step.  With
+        output_frequency 0 no step is an output step (the reference then
+        writes only its initial record), so the callback is not invoked."""
+        if callback is None or self.output_frequency == 0:
             return self.Advance(self.num_load_steps - self.step)
         while self.step < self.num_load_steps:
             nxt = self.step
-            while not self.is_output_step(nxt):
+            # the last load step is always an output step, so this ends
+            while nxt < self.num_load_steps - 1 and not self.is_output_step(nxt):
                 nxt += 1
             self.Advance(nxt + 1 - self.step)
             callback(nxt, self.time_current)

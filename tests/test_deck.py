@@ -159,3 +159,57 @@ def test_exodus_uq_sample_fields(tmp_path):
         assert np.array_equal(col["exact_displacement_2_y"], k + 20.0)
         assert np.array_equal(col["sample_force_1_z"], k + 31.0)
         assert np.array_equal(col["sample_displacement_3_x"], k + 50.0)
+
+
+REF_TEST = "/root/reference/test"
+REF_DECKS = {  # fixture name -> (directory, mesh, deck) under the reference's test tree
+    "single_elem_native_neohookean": ("dynamics/single_elem_complex_displacement", "single_elem.g",
+                                      "single_elem_native_neohookean.in"),
+    "simple_deformation_modes": ("dynamics/simple_deformation_modes", "simple_deformation_modes.g",
+                                 "simple_deformation_modes.in"),
+    "rigid_body_motion": ("dynamics/rigid_body_motion", "rigid_body_motion.g", "rigid_body_motion.in"),
+    "wave_in_bar": ("dynamics/wave_in_bar", "wave_in_bar.g", "wave_in_bar.in"),
+    "brick_with_fibers": ("dynamics/brick_with_fibers", "brick_with_fibers.g", "brick_with_fibers.in"),
+    "notched_plate_native_neohookean": ("dynamics/notched_plate_native_neohookean",
+                                        "notched_plate_native_neohookean.g", "notched_plate_native_neohookean.in"),
+    "uq_wave": ("uq/uq_wave", "uq_wave.g", "uq_wave.in"),
+}
+
+
+@pytest.mark.skipif(not os.path.isdir(REF_TEST), reason="NimbleSM test tree not present")
+@pytest.mark.parametrize("name", sorted(REF_DECKS))
+def test_read_reference_genesis_meshes(name):
+    """read_genesis on the reference's own meshes (src/nimble_genesis_mesh.cc:63-317
+    reads them through Exodus II) and parse_input_deck on its decks agree
+    with the committed fixtures, which a separate reader
+    (tests/golden/make_golden.py) extracted: coordinates and connectivity bit
+    for bit, blocks in ascending id with their element offsets, and the
+    deck's conditions expanded over the mesh's node sets in deck order."""
+    from tests.conftest import load_golden
+
+    sub, mesh_name, deck_name = REF_DECKS[name]
+    d = os.path.join(REF_TEST, sub)
+    mesh = D.read_genesis(os.path.join(d, mesh_name))
+    g = load_golden(name)
+    assert np.array_equal(mesh.X, g["X"])
+    ids = sorted(mesh.blocks)
+    assert ids == [int(b) for b in g["blk_ids"]]
+    off = np.concatenate([[0], np.cumsum([len(mesh.blocks[b]) for b in ids])])
+    assert np.array_equal(off, g["blk_off"])
+    assert np.array_equal(np.concatenate([mesh.blocks[b] for b in ids]), g["conn"])
+    deck = D.parse_input_deck(os.path.join(d, deck_name))
+    assert deck.genesis_input_file == mesh_name
+    kinds = {k: i for i, k in enumerate(D.BC_KINDS)}
+    bn, bc, bk, bv = [], [], [], []
+    for kind, ns, coord, mag in deck.boundary_conditions:
+        nodes = mesh.node_sets[ns]
+        bn += list(nodes)
+        bc += [coord] * len(nodes)
+        bk += [kinds[kind]] * len(nodes)
+        bv += [mag] * len(nodes)
+    assert np.array_equal(np.array(bn, np.int32), g["bc_node"])
+    assert np.array_equal(np.array(bc, np.int32), g["bc_coord"])
+    assert np.array_equal(np.array(bk, np.int32), g["bc_kind"])
+    assert np.array_equal(np.array(bv), g["bc_value"])
+    assert deck.number_of_load_steps == int(g["num_load_steps"])
+    assert deck.final_time == float(g["t_final"])

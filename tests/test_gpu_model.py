@@ -173,25 +173,122 @@ def test_mixed_runs_are_deterministic(hip_device):
 
 
 def test_large_cube_properties(hip_device):
-    """128^3, two blocks (the benchmark mesh): size-independent checks."""
+    """128^3, two blocks (the benchmark mesh), its full 500 steps:
+    size-independent checks, and mixed precision against the FP64 run on the
+    device: u <= 1e-6, v, f_int and the reaction history <= 1e-5 (norm-wise
+    relative); u and v also over the nodes without prescribed conditions."""
     prob = N.workload("cube128_2blk")
-    steps = 30
+    steps = prob.num_steps
     res = {}
     for prec in ("fp64", "mixed"):
+        print(f"cube128_2blk {prec}: {steps} steps", flush=True)
         m, integ = run_gpu_problem(prob, steps, prec)
         f = m.GetVectorNodeData("internal_force")
         u = m.GetVectorNodeData("displacement")
+        v = m.GetVectorNodeData("velocity")
         R = m.GetReactionHistory(0, steps)
         # translation invariance: internal forces sum to zero
         assert np.abs(f.sum(axis=0)).max() <= 1e-9 * np.abs(f).sum(axis=0).max()
         # prescribed top velocity: u_z on the top face = v_top * t exactly as accumulated
         assert np.allclose(u[prob.top, 2], integ.time_current * prob.v_top, rtol=1e-12)
         assert np.all(u[prob.bottom] == 0.0)
-        res[prec] = (u, f, R)
+        res[prec] = (u, v, f, R)
         m.close()
-    assert rel_err(res["mixed"][0], res["fp64"][0]) <= 1e-6
-    assert rel_err(res["mixed"][1], res["fp64"][1]) <= 1e-5
-    assert rel_err(res["mixed"][2], res["fp64"][2]) <= 1e-5
+        del m
+    free = np.ones(prob.num_nodes, bool)
+    free[prob.top] = free[prob.bottom] = False
+    errs = {name: rel_err(res["mixed"][i], res["fp64"][i]) for i, name in enumerate(("u", "v", "f", "R"))}
+    errs["u_free"] = rel_err(res["mixed"][0][free], res["fp64"][0][free])
+    errs["v_free"] = rel_err(res["mixed"][1][free], res["fp64"][1][free])
+    print("cube128_2blk mixed vs fp64 GPU after %d steps: " % steps +
+          " ".join(f"{k} {e:.2e}" for k, e in errs.items()), flush=True)
+    assert errs["u"] <= 1e-6 and errs["u_free"] <= 1e-6
+    assert errs["v"] <= 1e-5 and errs["v_free"] <= 1e-5
+    assert errs["f"] <= 1e-5 and errs["R"] <= 1e-5
+
+
+@pytest.mark.parametrize("case", ["uniaxial", "simple shear", "general"])
+def test_reaction_closed_form_on_device(hip_device, case):
+    """One device step from a uniform affine state x = F X of an undistorted
+    cube at rest: the top-face reaction equals the traction resultant
+    e_z . sigma(F) . J F^{-T} e_Z (the oracle's stress) to 1e-12 of its
+    magnitude in FP64 and 1e-5 in mixed precision, and the bottom face's is
+    its negative (one material; the same closed form pins the oracle)."""
+    from tests.test_oracle_math import AFFINE, STEEL_KG, closed_form_reaction, traction_resultant
+
+    F = AFFINE[case]
+    n = 8
+    X, conn = N.jittered_cube(n, 1, 0.0)
+    u = X @ (F - np.eye(3)).T
+    n1 = n + 1
+    top = (np.arange(n1 * n1) + n * n1 * n1).astype(np.int32)
+    bottom = np.arange(n1 * n1, dtype=np.int32)
+    scale = np.linalg.norm(traction_resultant(F, STEEL_KG[1], STEEL_KG[2]))
+    for precision, tol in (("fp64", 1e-12), ("mixed", 1e-5)):
+        for nodes, axis, sign in ((top, 2, 1.0), (bottom, 2, -1.0), (top, 0, 1.0)):
+            m = N.ModelData(X, [N.Block(1, conn, N.NeohookeanMaterial(*STEEL_KG))], precision)
+            m.SetReactionNodes(nodes, axis)
+            m.SetVectorNodeData("displacement", u)
+            m.ComputeLumpedMass()
+            m.ExplicitSteps(0, 1, 1, 0, 1e-9, 0.0)
+            R = m.GetReactionHistory(0, 1)[0]
+            ref = sign * closed_form_reaction(F, STEEL_KG[1], STEEL_KG[2], axis)
+            assert abs(R - ref) <= tol * scale, (precision, case, axis, R, ref)
+            m.close()
+
+
+def test_velocity_then_displacement_condition_on_one_node(hip_device):
+    """A prescribed displacement listed after a prescribed velocity on the
+    same node and coordinate: the reference applies both in order
+    (src/nimble_boundary_condition_manager.h:136-217), so at delta_t = 0 the
+    velocity condition holds, and UQ samples (which take velocities only)
+    keep it for the whole run.  FP64 device = oracle, bit for bit."""
+    prob = N.CubeProblem(n=4, materials=[N.problems.STEEL], num_steps=12, v_top=0.0)
+    prob.build()
+    side = np.arange(0, prob.num_nodes, 7, dtype=np.int32)  # some nodes of every layer
+    bcs = prob.bc_list() + [("prescribed_velocity", side, 0, 0.75), ("prescribed_displacement", side, 0, 2e-6),
+                            ("prescribed_velocity", side, 1, -0.5)]
+    kinds = {"initial_velocity": 0, "prescribed_velocity": 1, "prescribed_displacement": 2}
+
+    def gpu_model():
+        m = N.ModelData(prob.X, prob.blocks(), "fp64")
+        for kind, nodes, c, mag in bcs:
+            m.AddBoundaryCondition(kind, nodes, c, mag)
+        m.SetReactionNodes(prob.top, 2)
+        return m
+
+    om = oracle_model_from_problem(prob)
+    bn, bc, bk, bv = [], [], [], []
+    for kind, nodes, c, mag in bcs:
+        for nd in nodes:
+            bn.append(nd), bc.append(c), bk.append(kinds[kind]), bv.append(mag)
+    om = O.Model(om.X, om.conn, om.blk_off, om.rho, om.K, om.G, np.array(bn, np.int32), np.array(bc, np.int32),
+                 np.array(bk, np.int32), np.array(bv))
+    m = gpu_model()
+    integ = N.ExplicitTimeIntegrator(m, 12 * 1e-7, 12, 5).Initialize()
+    v0 = m.GetVectorNodeData("velocity")
+    assert np.all(v0[side, 0] == 0.75) and np.all(v0[side, 1] == -0.5)
+    run = O.ExplicitRun(om, 12 * 1e-7, 12, 5)
+    assert np.array_equal(v0, run.v)
+    integ.Integrate()
+    run.advance(12)
+    for label, ref in (("displacement", run.u), ("velocity", run.v), ("internal_force", run.f)):
+        assert np.array_equal(m.GetVectorNodeData(label), ref), label
+    m.close()
+    # UQ: one exact sample with the nominal moduli and one approximate sample
+    m = gpu_model()
+    K, G = prob.blocks()[0].material.bulk_modulus, prob.blocks()[0].material.shear_modulus
+    m.SetUqSamples([[K]], [[G]], [[0.0, 1.0]])
+    integ = N.ExplicitTimeIntegrator(m, 12 * 1e-7, 12, 5).Initialize()
+    integ.Integrate()
+    uq = O.UqRun(om, [[K]], [[G]], [[0.0, 1.0]], 12 * 1e-7, 12, 5)
+    uq.advance(12)
+    for s in range(2):
+        ru, rv, rf = uq.sample(s)
+        assert np.all(m.GetUqNodeData(s, "velocity")[side, 0] == 0.75)
+        for label, ref in (("displacement", ru), ("velocity", rv), ("internal_force", rf)):
+            assert np.array_equal(m.GetUqNodeData(s, label), ref), (s, label)
+    m.close()
 
 
 def test_largest_cube_mixed_against_fp64_gpu(hip_device):

@@ -98,3 +98,67 @@ def test_inverted_element_reports_error(hip_device):
                                         torch.zeros((1, 8, 3), dtype=torch.float64, device=hip_device))
     assert ei.value.status == 3
     N.HexElement.ComputeDeformationGradients(_dev(X[None], hip_device), _dev(np.zeros_like(u)[None], hip_device), F)
+
+
+def test_stress_strided_layouts(hip_device, elems):
+    """The strided stress seam on LayoutRight [E][8][9] and LayoutLeft
+    (element-fastest) arrays gives the oracle's stress bit for bit."""
+    import torch
+
+    X, u = elems
+    Fo = O.deformation_gradients(X, u)
+    E = len(X)
+    K, G = 166.67e9, 76.92e9
+    ref = O.stress(K, G, Fo)
+    mat = N.NeohookeanMaterial(7800.0, K, G)
+    # LayoutRight: (e, q, c) at 72 e + 9 q + c
+    Fr = _dev(Fo, hip_device)
+    Sr = torch.zeros((E, 8, 6), dtype=torch.float64, device=hip_device)
+    mat.GetStressStrided(E, 8, Fr.data_ptr(), (72, 9, 1), Sr.data_ptr(), (48, 6, 1))
+    # LayoutLeft: (e, q, c) at e + E q + 8 E c
+    Fl = _dev(np.ascontiguousarray(Fo.transpose(2, 1, 0)), hip_device)
+    Sl = torch.zeros((6, 8, E), dtype=torch.float64, device=hip_device)
+    mat.GetStressStrided(E, 8, Fl.data_ptr(), (1, E, 8 * E), Sl.data_ptr(), (1, E, 8 * E))
+    torch.cuda.synchronize()
+    assert np.array_equal(Sr.cpu().numpy(), ref)
+    assert np.array_equal(Sl.cpu().numpy().transpose(2, 1, 0), ref)
+
+
+def test_split_force_evaluation_matches_fused(hip_device):
+    """ComputeInternalForce split around a stress plug-in (F_np1 on the
+    device, the Neo-Hookean stress of each block through the strided seam on
+    the model's own arrays and stream, then the forces from that stress) is
+    bit-identical to the fused evaluation, and to the oracle."""
+    from tests.helpers import oracle_model_from_problem
+
+    prob = N.CubeProblem(n=9, materials=[N.problems.STEEL, N.problems.ALUMINIUM, N.problems.POLYMER],
+                         num_steps=1, seed=5)
+    rng = np.random.default_rng(4)
+    u = rng.normal(0, 2e-4, (prob.num_nodes, 3))
+    fused = prob.make_model("fp64")
+    fused.SetVectorNodeData("displacement", u)
+    fused.ComputeInternalForce()
+    split = prob.make_model("fp64")
+    split.SetVectorNodeData("displacement", u)
+    split.ComputeDeformationGradients()
+    Fp, eb = split.DevicePointer("deformation_gradient")
+    Sp, _ = split.DevicePointer("stress")
+    assert eb == 8
+    E = split.num_elem
+    for b, blk in enumerate(split.blocks):
+        e0, e1 = int(split.block_offsets[b]), int(split.block_offsets[b + 1])
+        blk.material.GetStressStrided(e1 - e0, 8, Fp + 8 * e0, (1, E, 8 * E), Sp + 8 * e0, (1, E, 8 * E),
+                                      stream=split.Stream())
+    split.ComputeInternalForceFromStress()
+    om = oracle_model_from_problem(prob)
+    f, Fip, sip = om.internal_force(u, want_ip=True)
+    for m in (fused, split):
+        assert np.array_equal(m.GetVectorNodeData("internal_force"), f)
+        assert np.array_equal(m.GetIntegrationPointData("deformation_gradient"), Fip)
+        assert np.array_equal(m.GetIntegrationPointData("stress"), sip)
+    mixed = prob.make_model("mixed")
+    with pytest.raises(N.NimbleError) as ei:
+        mixed.ComputeDeformationGradients()
+    assert ei.value.status == 4
+    for m in (fused, split, mixed):
+        m.close()

@@ -231,3 +231,97 @@ def test_cbrt_method_matches_platform_libm():
     assert np.array_equal(got, ref)
     # and the method is not the correctly rounded cube root
     assert np.sum(got[:20000] != np.cbrt(xs[:20000])) > 1000
+
+
+# ------------------------------------------------------------- reaction sum
+def affine_cube(n, F, materials):
+    """Undistorted n^3 unit cube (no jitter) with blocks in equal z-layers,
+    displacement u = (F - I) X, no conditions: the first step leaves u as it
+    is (v = a = 0), so the step's internal force is that of x = F X."""
+    X, conn = O.jittered_cube(n, 1, 0.0)
+    nb = len(materials)
+    per = n * n
+    off = [(n * b // nb) * per for b in range(nb + 1)]
+    rho = [m[0] for m in materials]
+    Kb = [m[1] for m in materials]
+    Gb = [m[2] for m in materials]
+    M = O.Model(X, conn, off, rho, Kb, Gb)
+    u = X @ (F - np.eye(3)).T
+    n1 = n + 1
+    layer = lambda k: np.arange(n1 * n1, dtype=np.int32) + k * n1 * n1
+    return M, u, layer(n), layer(0)
+
+
+def oracle_reaction(M, u, nodes, coord):
+    """R = -sum_{nodes} f_int[coord] of the oracle's first step from x = F X."""
+    r = O.ExplicitRun(M, 1e-9, 1, 0, reaction_nodes=nodes, reaction_coord=coord)
+    r.u[:] = u
+    r.advance(1)
+    assert np.array_equal(r.u, u)  # the state the force was evaluated at
+    return r.reaction[0]
+
+
+def traction_resultant(F, Kb, Gb):
+    S = sym_to_mat(O.stress(Kb, Gb, mat_to_kf(F)))
+    cof = np.linalg.det(F) * np.linalg.inv(F).T
+    return S @ cof @ np.array([0.0, 0.0, 1.0])
+
+
+def closed_form_reaction(F, Kb, Gb, axis):
+    """Traction resultant on the face Z = 1 of the unit cube under x = F X
+    with the oracle's (uniform) stress: Nanson's n da = J F^{-T} N dA with
+    N = e_Z, A = 1, so R = e_axis . sigma . J F^{-T} e_Z.  This equals
+    -sum_{top nodes} f_int,axis exactly for an undistorted mesh: the nodal
+    sum of B^T sigma is sigma . int grad(phi) dv with phi the sum of the
+    top nodes' shape functions (1 on Z = 1, 0 below the top element layer),
+    which 2x2x2 Gauss integration evaluates exactly."""
+    return traction_resultant(F, Kb, Gb)[axis]
+
+
+AFFINE = {
+    "uniaxial": np.diag([1.0, 1.0, 1.0015]),
+    "simple shear": np.array([[1.0, 0.0, 0.02], [0.0, 1.0, 0.0], [0.0, 0.0, 1.0]]),
+    "general": np.array([[1.0010, 0.0003, 0.0002], [-0.0001, 0.9995, 0.0004], [0.0002, -0.0003, 1.0020]]),
+}
+STEEL_KG = (7800.0, 166.67e9, 76.92e9)
+ALU_KG = (2700.0, 68.63e9, 26.32e9)
+
+
+@pytest.mark.parametrize("case", sorted(AFFINE))
+def test_reaction_sum_closed_form(case):
+    """The reaction history's sum (R = -sum_top f_int,z, a quantity the
+    reference does not output) against the traction resultant of a uniform
+    affine state: R_top = e_z . sigma(F) . J F^{-T} e_Z, and R_bottom =
+    -R_top for one material, to 1e-12 of the traction resultant's magnitude
+    (roundoff scales with the whole stress: under simple shear R_z =
+    -G gamma^2 / 3 is 1/150 of the shear traction)."""
+    F = AFFINE[case]
+    M, u, top, bottom = affine_cube(4, F, [STEEL_KG])
+    scale = np.linalg.norm(traction_resultant(F, STEEL_KG[1], STEEL_KG[2]))
+    for axis in (2, 0):
+        R = oracle_reaction(M, u, top, axis)
+        ref = closed_form_reaction(F, STEEL_KG[1], STEEL_KG[2], axis)
+        assert abs(R - ref) <= 1e-12 * scale, (axis, R, ref)
+        Rb = oracle_reaction(M, u, bottom, axis)
+        assert abs(Rb + R) <= 1e-12 * scale, (axis, Rb, R)
+    assert abs(closed_form_reaction(F, STEEL_KG[1], STEEL_KG[2], 2)) > 1e-4 * scale
+
+
+def test_reaction_sum_planted_mistakes_fail():
+    """A sign flip, the bottom face instead of the top, the x axis instead of
+    z, and the lower block's material all miss the closed form of the top
+    face by far more than its tolerance (two blocks: steel below, aluminium
+    on top; general affine F)."""
+    F = AFFINE["general"]
+    M, u, top, bottom = affine_cube(4, F, [STEEL_KG, ALU_KG])
+    ref = closed_form_reaction(F, ALU_KG[1], ALU_KG[2], 2)
+    good = oracle_reaction(M, u, top, 2)
+    assert abs(good - ref) <= 1e-12 * abs(ref)
+    planted = {
+        "sign flip": -good,
+        "bottom face": oracle_reaction(M, u, bottom, 2),
+        "x axis": oracle_reaction(M, u, top, 0),
+        "lower block's material": closed_form_reaction(F, STEEL_KG[1], STEEL_KG[2], 2),
+    }
+    for name, val in planted.items():
+        assert abs(val - ref) > 1e-3 * abs(ref), name
