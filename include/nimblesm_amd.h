@@ -195,6 +195,17 @@ int nsm_model_get_reaction_history(nsm_model* model, int64_t first_step, int64_t
  * linearly interpolated between integer steps j (value after step j-1;
  * step 0 is the model at rest, value 0).  out: num_samples doubles (host). */
 int nsm_model_sample_reaction_history(nsm_model* model, int64_t num_steps, int num_samples, double* out);
+/* The same sampling, allocating nothing: no counterpart in the reference.  For
+ * callers that cannot allocate device memory (code under stream capture,
+ * tight serving loops, harnesses that forbid hipMalloc), where
+ * nsm_model_sample_reaction_history allocates a device staging buffer per
+ * call.  The values are staged in a fixed device array of
+ * NSM_NOALLOC_MAX_SAMPLES doubles that each GPU holds for the life of the
+ * process; num_samples above that maximum fails with NSM_ERR_ARG.  All calls
+ * on one GPU share that array: calls that overlap on one GPU (other threads,
+ * other models' streams) race. */
+#define NSM_NOALLOC_MAX_SAMPLES 4096
+int nsm_model_sample_reaction_history_noalloc(nsm_model* model, int64_t num_steps, int num_samples, double* out);
 
 /* Steps [step_begin, step_end) of ExplicitTimeIntegrator::Integrate with
  * t += dt_user, delta = t - t_prev; *time_current is read and updated.  All

@@ -13,6 +13,7 @@ LIB_PATH = os.environ.get("NSM_LIBRARY") or os.path.join(HERE, "libnimblesm_amd.
 
 NSM_OK, NSM_ERR_ARG, NSM_ERR_HIP, NSM_ERR_NEG_JACOBIAN, NSM_ERR_STATE, NSM_ERR_NOMEM = range(6)
 NSM_FP64, NSM_MIXED = 0, 1
+NSM_NOALLOC_MAX_SAMPLES = 4096  # include/nimblesm_amd.h
 FIELDS = {
     "reference_coordinate": 0,
     "displacement": 1,
@@ -53,6 +54,7 @@ SIGNATURES = {
     "nsm_model_set_reaction_nodes": (_I, [_P, _I64, _P, _I]),
     "nsm_model_get_reaction_history": (_I, [_P, _I64, _I64, _P]),
     "nsm_model_sample_reaction_history": (_I, [_P, _I64, _I, _P]),
+    "nsm_model_sample_reaction_history_noalloc": (_I, [_P, _I64, _I, _P]),
     "nsm_model_explicit_steps": (_I, [_P, _I64, _I64, _I64, _I64, _D, C.POINTER(_D)]),
     "nsm_model_get_field": (_I, [_P, _I, _P]),
     "nsm_model_set_field": (_I, [_P, _I, _P]),

@@ -1248,6 +1248,66 @@ int nsm_model_sample_reaction_history(nsm_model* md, int64_t num_steps, int num_
   return NSM_OK;
 }
 
+}  // extern "C"
+
+namespace {
+
+// Staging of nsm_model_sample_reaction_history_noalloc: the sampled values
+// on their way to the host.  Every GPU the library runs on holds its own copy
+// (NSM_NOALLOC_MAX_SAMPLES doubles, 32 KiB) until the process ends.  In host
+// code `scratch` names a host-side stand-in, never the device memory: the
+// device address comes from hipGetSymbolAddress, once per device.
+__device__ __align__(256) double scratch[NSM_NOALLOC_MAX_SAMPLES];
+
+std::mutex            g_scratch_mutex;
+std::map<int, double*> g_scratch_ptr;  // device -> address of that device's scratch
+
+// Device address of `scratch` on the current device (looked up on the first
+// call on that device, with it current).
+int scratch_address(double** out)
+{
+  int dev = 0;
+  NSM_HIP(hipGetDevice(&dev));
+  std::lock_guard<std::mutex> lk(g_scratch_mutex);
+  auto                        it = g_scratch_ptr.find(dev);
+  if (it == g_scratch_ptr.end()) {
+    double* ptr = nullptr;
+    NSM_HIP(hipGetSymbolAddress((void**)&ptr, HIP_SYMBOL(scratch)));
+    if (!ptr) return fail(NSM_ERR_HIP, "hipGetSymbolAddress returned no address for the sampling scratch");
+    it = g_scratch_ptr.emplace(dev, ptr).first;
+  }
+  *out = it->second;
+  return NSM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int nsm_model_sample_reaction_history_noalloc(nsm_model* md, int64_t num_steps, int num_samples, double* out)
+{
+  if (!md || (num_samples > 0 && !out)) return fail(NSM_ERR_ARG, "null argument");
+  if (num_steps < 1 || num_steps > md->reaction_cap)
+    return fail(NSM_ERR_ARG, "reaction history of %lld steps not recorded", (long long)num_steps);
+  if (num_samples < 0) return fail(NSM_ERR_ARG, "negative sample count");
+  if (num_samples > NSM_NOALLOC_MAX_SAMPLES)
+    return fail(NSM_ERR_ARG, "%d samples exceed the allocation-free maximum of %d", num_samples,
+                NSM_NOALLOC_MAX_SAMPLES);
+  if (num_samples == 0) return NSM_OK;
+  int st = ensure_device(md);
+  if (st) return st;
+  double* d_out = nullptr;
+  if ((st = scratch_address(&d_out))) return st;
+  // the kernel writes every one of the num_samples values: nothing to clear
+  nsm::launch_sample_history(md->reaction_hist, num_steps, num_samples, d_out, md->stream);
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess) e = hipMemcpyAsync(out, d_out, (size_t)num_samples * sizeof(double), hipMemcpyDeviceToHost,
+                                          md->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(md->stream);
+  if (e != hipSuccess) return fail(NSM_ERR_HIP, "sampling the reaction history: %s", hipGetErrorString(e));
+  return NSM_OK;
+}
+
 int nsm_model_explicit_steps(nsm_model* md, int64_t step_begin, int64_t step_end, int64_t num_load_steps,
                              int64_t output_frequency, double dt_user, double* time_current)
 {
