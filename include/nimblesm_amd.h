@@ -291,6 +291,102 @@ int nsm_model_get_uq_field(nsm_model* model, int sample, int field, double* out)
 int nsm_model_set_uq_field(nsm_model* model, int sample, int field, const double* in);
 
 /* ---------------------------------------------------------------------------
+ * Batched ensemble: independent samples advanced together on one stream.
+ * Each sample is a copy of one of the ensemble's meshes with its own block
+ * materials (density, bulk and shear modulus), its own magnitudes of the
+ * mesh's boundary conditions and its own time step.  One step of every
+ * sample is one element launch over all samples (mixed precision: a
+ * persistent launch over the element tiles of every sample, which also
+ * updates tile-interior nodes; FP64: element chunks of every sample) and one
+ * node launch; a whole run can be captured in a HIP graph and replayed.
+ *
+ * The reference's counterpart is the serial loop over the exact samples of
+ * its UQ model (src/uq/nimble_uq_model_data.cc:145-248,
+ * src/uq/nimble_uq_block.cc:57-164), where all samples share one mesh, one
+ * lumped mass and one time step and differ only in K and G.  Here each sample
+ * has its own lumped mass (ModelData::ComputeLumpedMass with the sample's
+ * densities, src/nimble_kokkos_model_data.cc:1067-1152) and critical time
+ * step (src/nimble_block_base.cc:52-84), and its trajectory is the one the
+ * nsm_model_* entry points give for that sample alone (bit-identical in
+ * FP64; in mixed precision the same arithmetic, and every sample's result
+ * is independent of the other samples).
+ * ------------------------------------------------------------------------- */
+typedef struct nsm_ensemble nsm_ensemble;
+
+/* Kernels whose launch durations nsm_ensemble_kernel_time reports (eager
+ * runs with timing enabled). */
+typedef enum nsm_ensemble_kernel {
+  NSM_ENSEMBLE_KERNEL_ELEMENT = 0, /* element launch of a step (all samples)   */
+  NSM_ENSEMBLE_KERNEL_NODE    = 1  /* node launch of a step (all samples)      */
+} nsm_ensemble_kernel;
+
+/* Limits: conditions combine into at most NSM_ENSEMBLE_MAX_BC_CODES distinct
+ * per-node combinations per mesh; a mesh has at most 8 blocks. */
+#define NSM_ENSEMBLE_MAX_BC_CODES 8
+
+/* tile_elems: elements per element tile of the mixed-precision kernel (64,
+ * 128, 256 or 512), or 0 to let the autotuner choose at setup from timed
+ * launches (the choice is cached per ensemble shape for the process). */
+int nsm_ensemble_create(int precision, int tile_elems, nsm_ensemble** out);
+int nsm_ensemble_destroy(nsm_ensemble* ens);
+/* A mesh, with arguments as nsm_model_create's (blocks in storage order,
+ * conn concatenated); *mesh_id receives its index. */
+int nsm_ensemble_add_mesh(nsm_ensemble* ens, int64_t num_nodes, const double* reference_coordinates, int num_blocks,
+                          const int64_t* block_num_elem, const int* conn, int* mesh_id);
+/* A condition of a mesh, kinds as nsm_model_add_boundary_condition; the
+ * magnitude is given per sample (nsm_ensemble_add_samples), in the order the
+ * mesh's conditions are added, and conditions apply in that order. */
+int nsm_ensemble_add_boundary_condition(nsm_ensemble* ens, int mesh_id, int kind, int coordinate, int64_t num_nodes,
+                                        const int* node_ids);
+/* Reaction-force history of every sample of the mesh: -sum over node_ids of
+ * f_int[node][coordinate] after every step (as nsm_model_set_reaction_nodes). */
+int nsm_ensemble_set_reaction_nodes(nsm_ensemble* ens, int mesh_id, int64_t num_nodes, const int* node_ids,
+                                    int coordinate);
+/* num_samples samples of mesh mesh_id: density, bulk_modulus, shear_modulus
+ * [num_samples][num_blocks of the mesh]; bc_magnitude [num_samples][conditions
+ * of the mesh] (may be NULL when the mesh has none).  *first_sample receives
+ * the index of the first; samples are numbered in the order they are added.
+ * Before nsm_ensemble_setup only. */
+int nsm_ensemble_add_samples(nsm_ensemble* ens, int mesh_id, int num_samples, const double* density,
+                             const double* bulk_modulus, const double* shear_modulus, const double* bc_magnitude,
+                             int* first_sample);
+/* New parameters for samples [first_sample, first_sample + num_samples), all
+ * of one mesh, arrays as in nsm_ensemble_add_samples.  After setup: their
+ * lumped mass and critical time step are recomputed at once; the next run
+ * (or a replay of a captured run) uses them. */
+int nsm_ensemble_set_samples(nsm_ensemble* ens, int first_sample, int num_samples, const double* density,
+                             const double* bulk_modulus, const double* shear_modulus, const double* bc_magnitude);
+/* Device data of every sample, element tiles (mixed), and each sample's
+ * lumped mass and critical time step.  Once, after the last add_*. */
+int nsm_ensemble_setup(nsm_ensemble* ens);
+/* Each sample's critical time step (GetCriticalTimeStep), [num_samples]. */
+int nsm_ensemble_get_critical_time_steps(nsm_ensemble* ens, double* dt);
+/* Every sample from rest: u = v = a = 0, initial velocities, the kinematic
+ * conditions at t = 0, then num_steps steps of
+ * ExplicitTimeIntegrator::Integrate (src/integrators/explicit_time_integrator.cc:123-278)
+ * with dt_user = final_time[s] / num_steps, t += dt_user, on output steps as
+ * in nsm_model_explicit_steps (output_frequency 0: none).  use_graph: the run
+ * is captured once per (num_steps, output_frequency) into a HIP graph and
+ * replayed; per-sample parameters and times live in device memory, so a
+ * replay after nsm_ensemble_set_samples uses the new ones. */
+int nsm_ensemble_run(nsm_ensemble* ens, int64_t num_steps, int64_t output_frequency, const double* final_time,
+                     int use_graph);
+/* After a run: each sample's reaction history sampled at num_points equally
+ * spaced times (the rule of nsm_model_sample_reaction_history), out
+ * [num_samples][num_points]; and a sample's per-step history. */
+int nsm_ensemble_sample_reaction_history(nsm_ensemble* ens, int num_points, double* out);
+int nsm_ensemble_get_reaction_history(nsm_ensemble* ens, int sample, int64_t first_step, int64_t count, double* out);
+/* Host copy of a sample's node field (NSM_FIELD_DISPLACEMENT, _VELOCITY,
+ * _ACCELERATION, _INTERNAL_FORCE, _LUMPED_MASS) in its mesh's node order. */
+int nsm_ensemble_get_field(nsm_ensemble* ens, int sample, int field, double* out);
+/* Samples, nodes and elements over all samples, element tiles of all samples
+ * (mixed; element chunks in FP64), the tile size in use, device bytes. */
+int nsm_ensemble_info(nsm_ensemble* ens, int64_t* num_samples, int64_t* num_nodes, int64_t* num_elem,
+                      int64_t* work_items, int* tile_elems, int64_t* device_bytes);
+int nsm_ensemble_enable_kernel_timing(nsm_ensemble* ens, int enable);
+int nsm_ensemble_kernel_time(nsm_ensemble* ens, int kernel, double* avg_ms, int64_t* launches);
+
+/* ---------------------------------------------------------------------------
  * Synthetic input: n^3 HEX8 unit cube, node (i,j,k) -> i+(n+1)(j+(n+1)k),
  * interior nodes jittered by U(-f,f)/n per axis from SplitMix64 keyed by
  * (seed, 3*node+axis).  Host arrays X [(n+1)^3][3], conn [n^3][8].

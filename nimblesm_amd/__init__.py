@@ -13,16 +13,21 @@ from .model import (  # noqa: F401
     ModelData,
     NeohookeanMaterial,
 )
+from .ensemble import Ensemble, EnsembleProblem, config2_draws, draw_parameters  # noqa: F401
 from .problems import CubeProblem, jittered_cube, workload  # noqa: F401
 
 __all__ = [
     "Block",
     "CubeProblem",
+    "Ensemble",
+    "EnsembleProblem",
     "ExplicitTimeIntegrator",
     "HexElement",
     "ModelData",
     "NeohookeanMaterial",
     "NimbleError",
+    "config2_draws",
+    "draw_parameters",
     "jittered_cube",
     "lib",
     "workload",

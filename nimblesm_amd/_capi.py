@@ -70,6 +70,23 @@ SIGNATURES = {
     "nsm_model_assembly_info": (_I, [_P, C.POINTER(_I), C.POINTER(_I64), C.POINTER(_I64), C.POINTER(_I64),
                                        C.POINTER(_I64)]),
     "nsm_model_info": (_I, [_P, C.POINTER(_I64), C.POINTER(_I64), C.POINTER(_I64), C.POINTER(_I64)]),
+    "nsm_ensemble_create": (_I, [_I, _I, C.POINTER(_P)]),
+    "nsm_ensemble_destroy": (_I, [_P]),
+    "nsm_ensemble_add_mesh": (_I, [_P, _I64, _P, _I, _P, _P, C.POINTER(_I)]),
+    "nsm_ensemble_add_boundary_condition": (_I, [_P, _I, _I, _I, _I64, _P]),
+    "nsm_ensemble_set_reaction_nodes": (_I, [_P, _I, _I64, _P, _I]),
+    "nsm_ensemble_add_samples": (_I, [_P, _I, _I, _P, _P, _P, _P, C.POINTER(_I)]),
+    "nsm_ensemble_set_samples": (_I, [_P, _I, _I, _P, _P, _P, _P]),
+    "nsm_ensemble_setup": (_I, [_P]),
+    "nsm_ensemble_get_critical_time_steps": (_I, [_P, _P]),
+    "nsm_ensemble_run": (_I, [_P, _I64, _I64, _P, _I]),
+    "nsm_ensemble_sample_reaction_history": (_I, [_P, _I, _P]),
+    "nsm_ensemble_get_reaction_history": (_I, [_P, _I, _I64, _I64, _P]),
+    "nsm_ensemble_get_field": (_I, [_P, _I, _I, _P]),
+    "nsm_ensemble_info": (_I, [_P, C.POINTER(_I64), C.POINTER(_I64), C.POINTER(_I64), C.POINTER(_I64),
+                               C.POINTER(_I), C.POINTER(_I64)]),
+    "nsm_ensemble_enable_kernel_timing": (_I, [_P, _I]),
+    "nsm_ensemble_kernel_time": (_I, [_P, _I, C.POINTER(_D), C.POINTER(_I64)]),
     "nsm_jittered_cube": (_I, [_I, C.c_uint64, _D, _P, _P]),
 }
 

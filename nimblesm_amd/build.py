@@ -38,6 +38,7 @@ SOURCES = {
     "nsm_tiled.hip": ["-ffp-contract=fast-honor-pragmas"],
     "nsm_tiles.cpp": ["-ffp-contract=off", "-x", "hip"],
     "nsm_model.cpp": ["-ffp-contract=off", "-x", "hip"],
+    "nsm_ensemble.cpp": ["-ffp-contract=off", "-x", "hip"],
 }
 
 

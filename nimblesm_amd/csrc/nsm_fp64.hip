@@ -8,6 +8,7 @@
 // k of an element serves as node k (gather, force output) and as Gauss point
 // k (F, stress); LDS carries the per-element node data and per-point
 // inverse Jacobians between the two roles.
+#include "nsm_ensemble.h"
 #include "nsm_fp64_math.h"
 #include "nsm_kernels.h"
 
@@ -177,21 +178,14 @@ struct Lds {
 // kernel after a block material interface's ComputeStress,
 // src/nimble_kokkos_model_data.cc:1229-1257).
 template <int kMode>
-__global__ __launch_bounds__(kThreads) void k_element_fp64(const BlockTable<double> bt, int64_t E_total,
-                                                           const int* __restrict__ conn,
-                                                           const double* __restrict__ X,
-                                                           const double* __restrict__ u,
-                                                           double* __restrict__ Fip, double* __restrict__ sip,
-                                                           double* __restrict__ slots, int* __restrict__ status)
+__device__ __forceinline__ void element_fp64(Lds& lds, int64_t e, bool valid, double K, double G, int64_t E_total,
+                                             const int* __restrict__ conn, const double* __restrict__ X,
+                                             const double* __restrict__ u, double* __restrict__ Fip,
+                                             double* __restrict__ sip, double* __restrict__ slots,
+                                             int* __restrict__ status)
 {
-  __shared__ Lds lds;
-  const int     el    = threadIdx.x >> 3;
-  const int     k     = threadIdx.x & 7;
-  const int64_t e     = bt.off[0] + (int64_t)xcd_tile(blockIdx.x, gridDim.x) * kEPW + el;
-  const bool    valid = e < bt.off[bt.count];
-  const int     blk   = valid ? block_of(bt, e) : 0;
-  const double  K     = bt.K[blk], G = bt.G[blk];
-
+  const int el = threadIdx.x >> 3;
+  const int k  = threadIdx.x & 7;
   if (valid) {
     const int64_t nd = conn[e * 8 + k];
     lds.nodes.x[el][k][0] = X[3 * nd];
@@ -252,6 +246,42 @@ __global__ __launch_bounds__(kThreads) void k_element_fp64(const BlockTable<doub
     slots[(int64_t)(16 + k) * E_total + e] = f[2];
   }
   if (bad) atomicOr(status, 1);
+}
+
+template <int kMode>
+__global__ __launch_bounds__(kThreads) void k_element_fp64(const BlockTable<double> bt, int64_t E_total,
+                                                           const int* __restrict__ conn,
+                                                           const double* __restrict__ X,
+                                                           const double* __restrict__ u,
+                                                           double* __restrict__ Fip, double* __restrict__ sip,
+                                                           double* __restrict__ slots, int* __restrict__ status)
+{
+  __shared__ Lds lds;
+  const int     el    = threadIdx.x >> 3;
+  const int64_t e     = bt.off[0] + (int64_t)xcd_tile(blockIdx.x, gridDim.x) * kEPW + el;
+  const bool    valid = e < bt.off[bt.count];
+  const int     blk   = valid ? block_of(bt, e) : 0;
+  element_fp64<kMode>(lds, e, valid, bt.K[blk], bt.G[blk], E_total, conn, X, u, Fip, sip, slots, status);
+}
+
+// Ensemble: workgroup g takes chunk chunks[g] (at most kEPW elements of one
+// block of one sample); the sample's slots are [3][8][E_mesh] at slot_base.
+__global__ __launch_bounds__(kThreads) void k_ens_element_fp64(EnsNodes en, const EnsRange* __restrict__ chunks,
+                                                               const int* __restrict__ conn,
+                                                               const double* __restrict__ Kd,
+                                                               const double* __restrict__ Gd,
+                                                               double* __restrict__ slots, int* __restrict__ status)
+{
+  __shared__ Lds  lds;
+  const EnsRange  ch = ens_load(chunks, (int64_t)xcd_tile(blockIdx.x, gridDim.x));
+  const EnsSample sm = ens_load(en.samples, (int64_t)ch.s);
+  const EnsMesh   ms = ens_load(en.meshes, (int64_t)sm.mesh);
+  const int       el = threadIdx.x >> 3;
+  const int       kb = sm.kg_base + ch.block;
+  element_fp64<kElementFull>(lds, (int64_t)ch.first + el, el < ch.count, ens_load(Kd, (int64_t)kb),
+                             ens_load(Gd, (int64_t)kb), ms.num_elem, conn + 8 * (int64_t)ms.elem_base,
+                             en.X + 3 * (int64_t)sm.x_base, en.u + 3 * (int64_t)sm.node_base, nullptr, nullptr,
+                             slots + sm.slot_base, status);
 }
 
 // ---------------------------------------------------------------------------
@@ -446,6 +476,42 @@ __global__ __launch_bounds__(kThreads) void k_element_mass(int64_t e_begin, int6
   if (e < e_end) lumped_row(k, rho, det[el], &elem_mass[(int64_t)k * E_total + e]);
 }
 
+// Ensemble: element masses of chunk chunks[g] with its sample's block
+// density, into the sample's [8][E_mesh] rows at slot_base / 3.
+__global__ __launch_bounds__(kThreads) void k_ens_element_mass(EnsNodes en, const EnsRange* __restrict__ chunks,
+                                                               const int* __restrict__ conn,
+                                                               const double* __restrict__ rho,
+                                                               double* __restrict__ elem_mass,
+                                                               int* __restrict__ status)
+{
+  __shared__ double xs[kEPW][8][3], zs[kEPW][8][3], det[kEPW][8];
+  const EnsRange    ch = ens_load(chunks, (int64_t)blockIdx.x);
+  const EnsSample   sm = ens_load(en.samples, (int64_t)ch.s);
+  const EnsMesh     ms = ens_load(en.meshes, (int64_t)sm.mesh);
+  const int         el = threadIdx.x >> 3, k = threadIdx.x & 7;
+  const int64_t     e  = (int64_t)ch.first + el;
+  const bool        ok = el < ch.count;
+  const double*     X  = en.X + 3 * (int64_t)sm.x_base;
+  if (ok) {
+    const int64_t nd = conn[(ms.elem_base + e) * 8 + k];
+    for (int c = 0; c < 3; c++) {
+      xs[el][k][c] = X[3 * nd + c];
+      zs[el][k][c] = 0.0;
+    }
+  }
+  __syncthreads();
+  if (ok) {
+    double a[9], b[9], bi[9];
+    jacobians(k, xs[el], zs[el], a, b);
+    det[el][k] = invert3x3(b, bi);
+    if (det[el][k] <= 0.0) atomicOr(status, 1);
+  }
+  __syncthreads();
+  if (ok)
+    lumped_row(k, ens_load(rho, (int64_t)(sm.kg_base + ch.block)), det[el],
+               &elem_mass[sm.slot_base / 3 + (int64_t)k * ms.num_elem + e]);
+}
+
 // Minimum over the workgroup's elements of charlen / sound_speed on current
 // coordinates X + u (src/nimble_block_base.cc:52-84); wave-level shuffles
 // then LDS across the 4 waves.  min is exact in any order.
@@ -501,6 +567,22 @@ void launch_element_fp64(const BlockTable<double>& bt, int64_t E_total, const in
   else
     hipLaunchKernelGGL(k_element_fp64<kElementFull>, grid, block, 0, s, bt, E_total, conn, X, u, Fip, sip, slots,
                        status);
+}
+
+void launch_ens_element_fp64(const EnsNodes& en, const EnsRange* chunks, int num_chunks, const int* conn,
+                             const double* K, const double* G, double* slots, int* status, hipStream_t s)
+{
+  if (num_chunks <= 0) return;
+  hipLaunchKernelGGL(k_ens_element_fp64, dim3((unsigned)num_chunks), dim3(kThreads), 0, s, en, chunks, conn, K, G,
+                     slots, status);
+}
+
+void launch_ens_element_mass(const EnsNodes& en, const EnsRange* chunks, int num_chunks, const int* conn,
+                             const double* rho, double* elem_mass, int* status, hipStream_t s)
+{
+  if (num_chunks <= 0) return;
+  hipLaunchKernelGGL(k_ens_element_mass, dim3((unsigned)num_chunks), dim3(kThreads), 0, s, en, chunks, conn, rho,
+                     elem_mass, status);
 }
 
 void launch_seam_defgrad(int64_t E, const double* Xg, const double* ug, double* F, int* status, hipStream_t s)

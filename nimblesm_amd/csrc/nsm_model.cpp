@@ -42,6 +42,19 @@ int fail(int status, const char* fmt, ...)
   return status;
 }
 
+}  // namespace
+
+namespace nsm {
+// the calling thread's error message, shared with the other host files
+int set_last_error(int status, const char* msg)
+{
+  g_last_error = msg;
+  return status;
+}
+}  // namespace nsm
+
+namespace {
+
 #define NSM_HIP(call)                                                                              \
   do {                                                                                             \
     hipError_t _e = (call);                                                                        \

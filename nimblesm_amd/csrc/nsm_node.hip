@@ -8,6 +8,7 @@
 // adds element contributions in ascending element order starting from 0.0,
 // which is the serial scatter order of src/nimble_block.cc:374-383 and
 // src/nimble_model_data.cc:636-659.
+#include "nsm_ensemble.h"
 #include "nsm_node_step.h"
 
 namespace nsm {
@@ -47,6 +48,54 @@ __global__ __launch_bounds__(kNodeThreads) void k_prekick(NodeArrays na, StepTim
   store3(na.v, i, v);
 }
 
+// Internal force of a node from the element force slots of its incidences
+// [ptr[0], ptr[1]): ascending element order, starting from 0.0; incidences
+// are fetched 8 at a time so their loads overlap, then added in order.
+template <typename SlotT>
+__device__ __forceinline__ void gather_force(const uint32_t* ptr, const uint32_t* __restrict__ node_ent,
+                                             const SlotT* __restrict__ slots, int64_t stride, double f[3])
+{
+  const uint32_t p0 = ptr[0], p1 = ptr[1];
+  f[0] = f[1] = f[2] = 0.0;
+  for (uint32_t p = p0; p < p1; p += 8) {
+    const uint32_t n = p1 - p < 8u ? p1 - p : 8u;
+    uint32_t       ent[8];
+    SlotT          s0[8], s1[8], s2[8];
+#pragma unroll
+    for (uint32_t k = 0; k < 8; k++)
+      if (k < n) ent[k] = node_ent[p + k];
+#pragma unroll
+    for (uint32_t k = 0; k < 8; k++)
+      if (k < n) {
+        s0[k] = slots[ent[k]];
+        s1[k] = slots[stride + ent[k]];
+        s2[k] = slots[2 * stride + ent[k]];
+      }
+#pragma unroll
+    for (uint32_t k = 0; k < 8; k++)
+      if (k < n) {
+        f[0] += (double)s0[k];
+        f[1] += (double)s1[k];
+        f[2] += (double)s2[k];
+      }
+  }
+}
+
+// Partial reaction sum of a 256-node workgroup (fixed tree) when it holds
+// reaction nodes.
+__device__ __forceinline__ void node_tile_partial(double contrib, int member, double* out)
+{
+  if (!__syncthreads_or(member)) return;
+  __shared__ double red[kNodeThreads];
+  red[threadIdx.x] = contrib;
+  __syncthreads();
+  for (int w = kNodeThreads / 2; w > 0; w >>= 1) {
+    if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *out = red[0];
+}
+
 // Internal-force gather and the second half of step n
 // (src/integrators/explicit_time_integrator.cc:250-270), optionally fused
 // with the first half of step n+1, plus per-tile partial reaction sums over
@@ -66,32 +115,8 @@ __global__ __launch_bounds__(kNodeThreads) void k_node_update(NodeArrays na, con
   double         contrib = 0.0;
   int            member  = 0;  // node belongs to the reaction set
   if (i < na.num_nodes) {
-    // gather: ascending element order, starting from 0.0; incidences are
-    // fetched 8 at a time so their loads overlap, then added in order
-    const uint32_t p0 = na.node_ptr[i], p1 = na.node_ptr[i + 1];
-    double         f[3] = {0.0, 0.0, 0.0};
-    for (uint32_t p = p0; p < p1; p += 8) {
-      const uint32_t n = p1 - p < 8u ? p1 - p : 8u;
-      uint32_t       ent[8];
-      SlotT          s0[8], s1[8], s2[8];
-#pragma unroll
-      for (uint32_t k = 0; k < 8; k++)
-        if (k < n) ent[k] = na.node_ent[p + k];
-#pragma unroll
-      for (uint32_t k = 0; k < 8; k++)
-        if (k < n) {
-          s0[k] = slots[ent[k]];
-          s1[k] = slots[na.slot_stride + ent[k]];
-          s2[k] = slots[2 * na.slot_stride + ent[k]];
-        }
-#pragma unroll
-      for (uint32_t k = 0; k < 8; k++)
-        if (k < n) {
-          f[0] += (double)s0[k];
-          f[1] += (double)s1[k];
-          f[2] += (double)s2[k];
-        }
-    }
+    double f[3];
+    gather_force(na.node_ptr + i, na.node_ent, slots, na.slot_stride, f);
     const int flags = na.flags[i];
     if (rx.coord >= 0 && (flags & kReactionFlag)) {
       member  = 1;
@@ -103,16 +128,7 @@ __global__ __launch_bounds__(kNodeThreads) void k_node_update(NodeArrays na, con
   }
   if (rx.coord < 0) return;  // uniform across the grid
 
-  // partial of this node tile (fixed tree) when it holds reaction nodes
-  if (!__syncthreads_or(member)) return;
-  __shared__ double red[kNodeThreads];
-  red[threadIdx.x] = contrib;
-  __syncthreads();
-  for (int w = kNodeThreads / 2; w > 0; w >>= 1) {
-    if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) rx.partials[tile] = red[0];
+  node_tile_partial(contrib, member, rx.partials + tile);
 }
 
 __global__ __launch_bounds__(kNodeThreads) void k_reaction_finalize(const double* __restrict__ partials,
@@ -140,18 +156,23 @@ __global__ __launch_bounds__(kNodeThreads) void k_mass_gather(int64_t N, const u
 // p = (k+1) n / num_samples steps, linear between integer steps, where step
 // j >= 1 holds hist[j-1] (the total after step j-1) and step 0 is 0 (a model
 // at rest).
-__global__ __launch_bounds__(kNodeThreads) void k_sample_history(const double* __restrict__ hist, int64_t n,
-                                                                 int num_samples, double* __restrict__ out)
+__device__ __forceinline__ double sampled_history(const double* __restrict__ hist, int64_t n, int k, int num_samples)
 {
-  const int k = (int)(blockIdx.x * blockDim.x + threadIdx.x);
-  if (k >= num_samples) return;
   const double  p  = (double)(k + 1) * (double)n / (double)num_samples;
   int64_t       j  = (int64_t)p;  // p >= 0
   if (j > n) j = n;
   const double  w  = p - (double)j;
   const double  r0 = j == 0 ? 0.0 : hist[j - 1];
   const double  r1 = j < n ? hist[j] : r0;
-  out[k]           = (1.0 - w) * r0 + w * r1;
+  return (1.0 - w) * r0 + w * r1;
+}
+
+__global__ __launch_bounds__(kNodeThreads) void k_sample_history(const double* __restrict__ hist, int64_t n,
+                                                                 int num_samples, double* __restrict__ out)
+{
+  const int k = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  if (k >= num_samples) return;
+  out[k] = sampled_history(hist, n, k, num_samples);
 }
 
 // UQ closure (UqModel::ApplyClosure, src/uq/nimble_uq.cc:327-346) fused with
@@ -203,7 +224,220 @@ __global__ __launch_bounds__(kNodeThreads) void k_uq_closure(NodeArrays na, UqCl
   }
 }
 
+// ---------------------------------------------------------------------------
+// Ensemble node kernels: workgroup g takes chunks[g], at most kNodeThreads
+// consecutive nodes (rows local to the sample's mesh) of one sample.
+// ---------------------------------------------------------------------------
+struct EnsNode {
+  int64_t row, xrow;  // row of u, v, a, f, mass; row of X and flags
+  bool    valid;
+};
+
+__device__ __forceinline__ EnsNode ens_node(const EnsNodes& en, const EnsRange& ch, const EnsSample& sm)
+{
+  const int     l = ch.first + (int)threadIdx.x;
+  EnsNode       n;
+  n.valid = (int)threadIdx.x < ch.count;
+  n.row   = (int64_t)sm.node_base + l;
+  n.xrow  = (int64_t)sm.x_base + l;
+  return n;
+}
+
+__device__ __forceinline__ NodeArrays ens_arrays(const EnsNodes& en, const EnsSample& sm)
+{
+  NodeArrays na;
+  na.num_nodes   = 0;
+  na.slot_stride = 0;
+  na.node_ptr    = nullptr;
+  na.node_ent    = nullptr;
+  na.mass        = en.mass;
+  na.u           = en.u;
+  na.v           = en.v;
+  na.a           = en.a;
+  na.f           = en.f;
+  na.flags       = en.flags;
+  na.bc_table    = en.bc + sm.bc_base;
+  na.num_bc      = 0;
+  return na;
+}
+
+// every sample at rest: u = v = a = f = 0, then its initial velocities
+// (ModelDataBase::ApplyInitialConditions) and its kinematic conditions at
+// the initial time (ApplyKinematicConditions(t0, t0): delta t = 0)
+__global__ __launch_bounds__(kNodeThreads) void k_ens_reset(EnsNodes en, const EnsRange* __restrict__ chunks)
+{
+  const EnsRange  ch = ens_load(chunks, (int64_t)blockIdx.x);
+  const EnsSample sm = ens_load(en.samples, (int64_t)ch.s);
+  const EnsNode   n  = ens_node(en, ch, sm);
+  if (!n.valid) return;
+  double    u[3] = {0.0, 0.0, 0.0}, v[3] = {0.0, 0.0, 0.0};
+  const int code = en.flags[n.xrow] & 0x7f;
+  if (code) {
+    const BcEntry& ini = en.init[sm.bc_base + code - 1];
+#pragma unroll
+    for (int c = 0; c < 3; c++)
+      if (ini.kind[c]) v[c] = ini.value[c];
+    apply_bc(en.bc + sm.bc_base, code, v, u, 0.0);
+  }
+  store3(en.u, n.row, u);
+  store3(en.v, n.row, v);
+  store3(en.a, n.row, u);
+  store3(en.f, n.row, u);
+}
+
+// first half of step es.step (src/integrators/explicit_time_integrator.cc:197-216)
+__global__ __launch_bounds__(kNodeThreads) void k_ens_prekick(EnsNodes en, const EnsRange* __restrict__ chunks,
+                                                              EnsStep es)
+{
+  const EnsRange  ch = ens_load(chunks, (int64_t)blockIdx.x);
+  const EnsSample sm = ens_load(en.samples, (int64_t)ch.s);
+  const StepTimes t  = ens_step_args(es, ch.s).cur;
+  const EnsNode   n  = ens_node(en, ch, sm);
+  if (!n.valid) return;
+  const int code = en.flags[n.xrow] & 0x7f;
+  double    u[3], v[3], a[3];
+  load3(en.u, n.row, u);
+  load3(en.v, n.row, v);
+  load3(en.a, n.row, a);
+  const double delta_t = t.t_cur - t.t_prev;
+  kick(v, a, t.half_dt);
+  apply_bc(en.bc + sm.bc_base, code, v, u, delta_t);
+  kick(u, v, t.dt);
+  apply_bc(en.bc + sm.bc_base, code, v, u, delta_t);
+  store3(en.u, n.row, u);
+  store3(en.v, n.row, v);
+}
+
+// every sample's reaction total of fold.step, one wave per sample
+__device__ __forceinline__ void ens_fold_waves(const EnsFold& f, int first_wave, int wave_stride)
+{
+  if (f.step < 0) return;
+  for (int s = first_wave + (int)(threadIdx.x >> 6); s < f.num_samples; s += wave_stride) {
+    const int    p0    = f.ptr[s];
+    const double total = wave_reaction_total(f.partials, f.idx + p0, f.ptr[s + 1] - p0);
+    if ((threadIdx.x & 63) == 0) f.hist[(int64_t)s * f.hist_stride + f.step] = total;
+  }
+}
+
+__global__ __launch_bounds__(kNodeThreads) void k_ens_fold(EnsFold fold)
+{
+  constexpr int kWaves = kNodeThreads / 64;
+  ens_fold_waves(fold, (int)blockIdx.x * kWaves, (int)gridDim.x * kWaves);
+}
+
+// FP64 step of one chunk of nodes: the gather of k_node_update over the
+// sample's slots and its node update; a previous step's totals are folded
+// by the first waves of the launch.
+__global__ __launch_bounds__(kNodeThreads) void k_ens_node_fp64(EnsNodes en, const EnsRange* __restrict__ chunks,
+                                                                const uint32_t* __restrict__ node_ptr,
+                                                                const uint32_t* __restrict__ node_ent,
+                                                                const double* __restrict__ slots, EnsStep es,
+                                                                EnsReaction rx, EnsFold fold)
+{
+  constexpr int kWaves = kNodeThreads / 64;
+  ens_fold_waves(fold, (int)blockIdx.x * kWaves, (int)gridDim.x * kWaves);
+  const unsigned  g  = xcd_tile(blockIdx.x, gridDim.x);
+  const EnsRange  ch = ens_load(chunks, (int64_t)g);
+  const EnsSample sm = ens_load(en.samples, (int64_t)ch.s);
+  const EnsMesh   ms = ens_load(en.meshes, (int64_t)sm.mesh);
+  const StepArgs  sa = ens_step_args(es, ch.s);
+  const EnsNode   n  = ens_node(en, ch, sm);
+  double          contrib = 0.0;
+  int             member  = 0;
+  if (n.valid) {
+    double f[3];
+    gather_force(node_ptr + ms.ptr_base + ch.first + threadIdx.x, node_ent, slots + sm.slot_base,
+                 8 * (int64_t)ms.num_elem, f);
+    const int flags = en.flags[n.xrow];
+    if (rx.coord >= 0 && (flags & kReactionFlag)) {
+      member  = 1;
+      contrib = f[rx.coord];
+    }
+    const NodeArrays na = ens_arrays(en, sm);
+    double           u[3];
+    if (sa.update_v) load3(na.u, n.row, u);
+    node_step(na, n.row, flags, f, u, sa);
+  }
+  if (rx.coord < 0) return;  // uniform across the grid
+  node_tile_partial(contrib, member, rx.partials + g);
+}
+
+__global__ __launch_bounds__(kNodeThreads) void k_ens_mass_gather(EnsNodes en, const EnsRange* __restrict__ chunks,
+                                                                  const uint32_t* __restrict__ node_ptr,
+                                                                  const uint32_t* __restrict__ node_ent,
+                                                                  const double* __restrict__ elem_mass,
+                                                                  double* __restrict__ m)
+{
+  const EnsRange  ch = ens_load(chunks, (int64_t)blockIdx.x);
+  const EnsSample sm = ens_load(en.samples, (int64_t)ch.s);
+  const EnsMesh   ms = ens_load(en.meshes, (int64_t)sm.mesh);
+  const EnsNode   n  = ens_node(en, ch, sm);
+  if (!n.valid) return;
+  const uint32_t* ptr = node_ptr + ms.ptr_base + ch.first + threadIdx.x;
+  const double*   em  = elem_mass + sm.slot_base / 3;
+  double          s   = 0.0;
+  for (uint32_t p = ptr[0]; p < ptr[1]; ++p) s += em[node_ent[p]];
+  m[n.row] = s;
+}
+
+__global__ __launch_bounds__(kNodeThreads) void k_ens_sample_history(const double* __restrict__ hist,
+                                                                     int64_t hist_stride, int64_t n, int num_samples,
+                                                                     int num_points, double* __restrict__ out)
+{
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= (int64_t)num_samples * num_points) return;
+  const int s = (int)(k / num_points), p = (int)(k % num_points);
+  out[k] = sampled_history(hist + (int64_t)s * hist_stride, n, p, num_points);
+}
+
 }  // namespace
+
+void launch_ens_reset(const EnsNodes& en, const EnsRange* chunks, int num_chunks, EnsStep, hipStream_t s)
+{
+  if (num_chunks <= 0) return;
+  hipLaunchKernelGGL(k_ens_reset, dim3((unsigned)num_chunks), dim3(kNodeThreads), 0, s, en, chunks);
+}
+
+void launch_ens_prekick(const EnsNodes& en, const EnsRange* chunks, int num_chunks, EnsStep es, hipStream_t s)
+{
+  if (num_chunks <= 0) return;
+  hipLaunchKernelGGL(k_ens_prekick, dim3((unsigned)num_chunks), dim3(kNodeThreads), 0, s, en, chunks, es);
+}
+
+void launch_ens_node_fp64(const EnsNodes& en, const EnsRange* chunks, int num_chunks, const uint32_t* node_ptr,
+                          const uint32_t* node_ent, const double* slots, EnsStep es, EnsReaction rx, EnsFold fold,
+                          hipStream_t s)
+{
+  if (num_chunks <= 0) return;
+  hipLaunchKernelGGL(k_ens_node_fp64, dim3((unsigned)num_chunks), dim3(kNodeThreads), 0, s, en, chunks, node_ptr,
+                     node_ent, slots, es, rx, fold);
+}
+
+void launch_ens_fold(EnsFold fold, hipStream_t s)
+{
+  if (fold.step < 0 || fold.num_samples <= 0) return;
+  constexpr int kWaves = kNodeThreads / 64;
+  hipLaunchKernelGGL(k_ens_fold, dim3((unsigned)((fold.num_samples + kWaves - 1) / kWaves)), dim3(kNodeThreads), 0,
+                     s, fold);
+}
+
+void launch_ens_mass_gather(const EnsNodes& en, const EnsRange* chunks, int num_chunks, const uint32_t* node_ptr,
+                            const uint32_t* node_ent, const double* elem_mass, double* mass, hipStream_t s)
+{
+  if (num_chunks <= 0) return;
+  hipLaunchKernelGGL(k_ens_mass_gather, dim3((unsigned)num_chunks), dim3(kNodeThreads), 0, s, en, chunks, node_ptr,
+                     node_ent, elem_mass, mass);
+}
+
+void launch_ens_sample_history(const double* hist, int64_t hist_stride, int64_t n, int num_samples, int num_points,
+                               double* out, hipStream_t s)
+{
+  const int64_t total = (int64_t)num_samples * num_points;
+  if (total <= 0) return;
+  hipLaunchKernelGGL(k_ens_sample_history, dim3((unsigned)((total + kNodeThreads - 1) / kNodeThreads)),
+                     dim3(kNodeThreads), 0, s, hist, hist_stride, n, num_samples, num_points, out);
+}
+
 
 void launch_sample_history(const double* hist, int64_t n, int num_samples, double* out, hipStream_t s)
 {

@@ -125,6 +125,27 @@ __device__ __forceinline__ double reaction_total(const double* partials, const i
   return -red[0];
 }
 
+// reaction_total computed by one 64-lane wave, with the same additions in the
+// same order: lane l holds the 256-lane sums of lanes l, l + 64, l + 128,
+// l + 192, the first two tree levels (w = 128, 64) are added in registers
+// and the rest (w = 32 .. 1, red[v] += red[v + w]) by shuffles down.  Valid
+// in lane 0; no barrier, so any wave can fold any sample.
+__device__ __forceinline__ double wave_reaction_total(const double* partials, const int* idx, int n)
+{
+  const int lane = (int)(threadIdx.x & 63);
+  double    r[4];
+#pragma unroll
+  for (int j = 0; j < 4; j++) {
+    double s = 0.0;
+    for (int k = lane + 64 * j; k < n; k += 256) s += partials[idx[k]];
+    r[j] = s;
+  }
+  double v = (r[0] + r[2]) + (r[1] + r[3]);
+#pragma unroll
+  for (int w = 32; w > 0; w >>= 1) v += __shfl_down(v, w, 64);
+  return -v;
+}
+
 // node_update with v and m loaded from the node arrays.
 __device__ __forceinline__ void node_step(const NodeArrays& na, int64_t i, int flags, const double f[3],
                                           double u[3], const StepArgs& sa)

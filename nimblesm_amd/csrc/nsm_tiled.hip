@@ -24,6 +24,7 @@
 #include "nsm_tiles.h"
 
 #include <algorithm>
+#include <type_traits>
 
 namespace nsm {
 namespace {
@@ -92,17 +93,18 @@ struct SlotIds {
 };
 
 template <int kThreads>
-__device__ __forceinline__ SlotIds load_ids(const TileArgs& ta, int t)
+__device__ __forceinline__ SlotIds load_ids(const TileArgs& ta, const TileCtx& c)
 {
-  t            = uniform(t);
+  const int t  = uniform(c.t);
   const int n0 = tile_table(ta.node_off, t), nn = tile_table(ta.node_off, t + 1) - n0;
   SlotIds   r;
 #pragma unroll
   for (int k = 0; k < 2; k++) {
     const int ln = (int)threadIdx.x + k * kThreads;
     const int sl = n0 + min(ln, nn - 1);
-    r.gid[k]     = ta.nodes[sl];
-    r.dst[k]     = ln < nn ? ta.slot_dst[sl] : kAbsent;
+    r.gid[k]     = ta.nodes[sl] + c.node_base;
+    const int d  = ta.slot_dst[sl];
+    r.dst[k]     = ln < nn ? (d >= 0 ? d + c.partial_base : d) : kAbsent;
   }
   return r;
 }
@@ -156,9 +158,9 @@ struct NodeState {
 };
 
 template <int kThreads>
-__device__ __forceinline__ void stage(const SlotIds& ids, const double (&xr)[2][3], const double (&ur)[2][3],
-                                      const NodeArrays& na, const StepArgs& sa, double (*sxu)[6],
-                                      Assembly<kThreads>& as, bool gather, NodeState& ns)
+__device__ __forceinline__ void stage(const SlotIds& ids, int x_delta, const double (&xr)[2][3],
+                                      const double (&ur)[2][3], const NodeArrays& na, int update_v,
+                                      double (*sxu)[6], Assembly<kThreads>& as, bool gather, NodeState& ns)
 {
   if (gather && threadIdx.x < 24) as.g.ef[threadIdx.x][kThreads] = 0.0f;
 #pragma unroll
@@ -177,12 +179,81 @@ __device__ __forceinline__ void stage(const SlotIds& ids, const double (&xr)[2][
         for (int c = 0; c < 3; c++) as.acc[ln][c] = 0.0;
     }
     if (ids.dst[k] == kInterior) {
-      ns.flg[k] = na.flags[ids.gid[k]];
-      if (sa.update_v) {
+      ns.flg[k] = na.flags[ids.gid[k] + x_delta];
+      if (update_v) {
         load3(na.v, ids.gid[k], ns.v[k]);
         ns.m[k] = na.mass[ids.gid[k]];
       }
     }
+  }
+}
+
+// ---- the two kinds of element-launch work (nsm_tiled.h) ----
+__device__ __forceinline__ int work_count(const ModelTileWork& w) { return w.t_count; }
+__device__ __forceinline__ int work_count(const EnsTileWork& w) { return w.count; }
+__device__ __forceinline__ int work_update_v(const ModelTileWork& w) { return w.sa.update_v; }
+__device__ __forceinline__ int work_update_v(const EnsTileWork& w) { return w.es.update_v; }
+
+__device__ __forceinline__ TileCtx tile_ctx(const ModelTileWork& w, int i)
+{
+  return TileCtx{w.t_begin + i, 0, 0, 0, 0, 0, 0};
+}
+__device__ __forceinline__ TileCtx tile_ctx(const EnsTileWork& w, int i)
+{
+  const EnsTile   et = ens_load(w.tiles, (int64_t)uniform(i));
+  const EnsSample sm = ens_load(w.samples, (int64_t)uniform(et.s));
+  return TileCtx{uniform(et.t), uniform(et.s), uniform(sm.node_base), uniform(sm.x_base - sm.node_base),
+                 uniform(sm.partial_base), uniform(sm.bc_base), uniform(sm.kg_base)};
+}
+
+__device__ __forceinline__ const BcEntry* work_bc(const ModelTileWork&) { return nullptr; }
+__device__ __forceinline__ const BcEntry* work_bc(const EnsTileWork& w) { return w.bc; }
+
+// moduli of the tile's block (tiles never straddle blocks)
+__device__ __forceinline__ void tile_moduli(const ModelTileWork& w, const TileCtx&, int e0, float& K, float& G)
+{
+  const int b = uniform(block_of(w.bt, (int64_t)e0));
+  K           = w.bt.K[b];
+  G           = w.bt.G[b];
+}
+__device__ __forceinline__ void tile_moduli(const EnsTileWork& w, const TileCtx& c, int, float& K, float& G)
+{
+  const int b = c.kg_base + tile_table(w.tile_block, c.t);
+  K           = tile_table(w.K, b);
+  G           = tile_table(w.G, b);
+}
+
+__device__ __forceinline__ StepArgs tile_step(const ModelTileWork& w, const TileCtx&) { return w.sa; }
+__device__ __forceinline__ StepArgs tile_step(const EnsTileWork& w, const TileCtx& c)
+{
+  return ens_step_args(w.es, c.s);
+}
+
+// index of the tile's reaction partial
+__device__ __forceinline__ int tile_rx_slot(const ModelTileWork&, const TileCtx& c, int) { return c.t; }
+__device__ __forceinline__ int tile_rx_slot(const EnsTileWork&, const TileCtx&, int i) { return i; }
+
+// A previous step's reaction total(s): workgroup 0 sums the model's
+// partials; in an ensemble every wave of the launch takes samples in turn.
+template <int kThreads>
+__device__ __forceinline__ void fold_previous(const ModelTileWork& w, double* fold_red)
+{
+  if (w.fold.n > 0 && blockIdx.x == 0) {  // uniform
+    const double total = reaction_total<kThreads>(w.fold.partials, w.fold.tiles, w.fold.n, fold_red);
+    if (threadIdx.x == 0) *w.fold.out = total;
+  }
+}
+template <int kThreads>
+__device__ __forceinline__ void fold_previous(const EnsTileWork& w, double*)
+{
+  const EnsFold& f = w.fold;
+  if (f.step < 0) return;
+  constexpr int kWaves = kThreads / 64;
+  for (int s = (int)blockIdx.x * kWaves + (int)(threadIdx.x >> 6); s < f.num_samples;
+       s += (int)gridDim.x * kWaves) {
+    const int    p0    = f.ptr[s];
+    const double total = wave_reaction_total(f.partials, f.idx + p0, f.ptr[s + 1] - p0);
+    if ((threadIdx.x & 63) == 0) f.hist[(int64_t)s * f.hist_stride + f.step] = total;
   }
 }
 
@@ -200,15 +271,18 @@ __device__ __forceinline__ void stage(const SlotIds& ids, const double (&xr)[2][
 // has, and the kernel is no faster for the bytes it saves.  The launch bound
 // holds the kernel to two waves per SIMD (at most 256 registers): left free,
 // the scheduler takes more for the gather assembly and halves occupancy.
-template <int kThreads, bool kStoreIp>
-__global__ __launch_bounds__(kThreads, 2) void k_element_tiled(const BlockTable<float> bt,
-                                                            TileArgs ta, int t_begin, int t_count,
+template <int kThreads, bool kStoreIp, typename Work>
+__global__ __launch_bounds__(kThreads, 2) void k_element_tiled(const Work work, TileArgs ta,
                                                             const double* __restrict__ X, NodeArrays na,
                                                             float* __restrict__ Fip, float* __restrict__ sip,
-                                                            StepArgs sa, ReactionArgs rx, ReactionFold fold,
-                                                            int* __restrict__ status)
+                                                            ReactionArgs rx, int* __restrict__ status)
 {
-  constexpr int kNodes = 2 * kThreads;
+  constexpr int  kNodes      = 2 * kThreads;
+  constexpr bool kEnsemble   = std::is_same<Work, EnsTileWork>::value;
+  // ensembles stage each tile's sample's kinematic-condition entries (one
+  // 8-byte word per thread)
+  constexpr int  kTileBcWords = kBcWords * kEnsMaxBcCodes;
+  static_assert(!kEnsemble || kTileBcWords <= kThreads, "one condition word per thread");
   // X and u of a node slot side by side (48 bytes): an element reads each
   // of its nodes with three 16-byte LDS reads
   __shared__ __attribute__((aligned(16))) double sxu[kNodes][6];
@@ -218,60 +292,69 @@ __global__ __launch_bounds__(kThreads, 2) void k_element_tiled(const BlockTable<
   __shared__ int     bad;
   __shared__ double  fold_red[256];
 
-  if (fold.n > 0 && blockIdx.x == 0) {  // uniform
-    const double total = reaction_total<kThreads>(fold.partials, fold.tiles, fold.n, fold_red);
-    if (threadIdx.x == 0) *fold.out = total;
-  }
+  fold_previous<kThreads>(work, fold_red);
 
   // Workgroup w (numbered so that each XCD holds a contiguous run of w)
-  // processes tiles t_begin + w, + G, + 2G, ... (G workgroups): at any time
+  // processes work items w, w + G, w + 2G, ... (G workgroups): at any time
   // the grid works on one window of G consecutive tiles, so the F and sigma
   // rows are written in one narrow address range and each XCD's L2 sees
   // neighbouring tiles.
-  const int G    = (int)gridDim.x;
-  const int w    = (int)xcd_tile(blockIdx.x, gridDim.x);
-  const int tend = t_begin + t_count;
-  const int tb   = uniform(t_begin + w);
-  if (tb >= tend) return;  // uniform across the workgroup
-  // last tile of this workgroup, and the tile after t (t itself if none)
-  const int  te   = uniform(tb + ((tend - 1 - tb) / G) * G + 1);
-  const auto next = [&](int t) { return t + G < te ? t + G : t; };
+  const int G        = (int)gridDim.x;
+  const int w        = (int)xcd_tile(blockIdx.x, gridDim.x);
+  const int count    = work_count(work);
+  const int update_v = work_update_v(work);
+  const int ib       = uniform(w);
+  if (ib >= count) return;  // uniform across the workgroup
+  // last work item of this workgroup, and the item after i (i itself if none)
+  const int  ie   = uniform(ib + ((count - 1 - ib) / G) * G + 1);
+  const auto next = [&](int i) { return i + G < ie ? i + G : i; };
   const int  tid  = (int)threadIdx.x;
 
-  // ---- prologue: stage tile tb, load tile tb+1's node ids
+  // ---- prologue: stage item ib, load item ib+G's node ids
   if (tid == 0) bad = 0;
-  if (sa.update_v)
+  TileCtx c_first = tile_ctx(work, ib);
+  if (!kEnsemble && update_v)
     for (int b = tid; b < na.num_bc; b += kThreads) sbc[b] = na.bc_table[b];
-  SlotIds ids      = load_ids<kThreads>(ta, tb);
-  SlotIds ids_next = load_ids<kThreads>(ta, next(tb));
+  if (kEnsemble && tid < kTileBcWords)
+    reinterpret_cast<uint64_t*>(sbc)[tid] =
+        reinterpret_cast<const uint64_t*>(work_bc(work) + c_first.bc_base)[tid];
+  SlotIds ids      = load_ids<kThreads>(ta, c_first);
+  SlotIds ids_next = load_ids<kThreads>(ta, tile_ctx(work, next(ib)));
   double  xr[2][3], ur[2][3];
 #pragma unroll
   for (int k = 0; k < 2; k++) {
-    load3(X, ids.gid[k], xr[k]);
+    load3(X, ids.gid[k] + c_first.x_delta, xr[k]);
     load3(na.u, ids.gid[k], ur[k]);
   }
-  ElemData  cur = load_elem<kThreads>(ta, tb);
+  ElemData  cur = load_elem<kThreads>(ta, c_first.t);
   NodeState ns;
-  stage<kThreads>(ids, xr, ur, na, sa, sxu, as, tile_table(ta.gather, tb) != 0, ns);
+  stage<kThreads>(ids, c_first.x_delta, xr, ur, na, update_v, sxu, as, tile_table(ta.gather, c_first.t) != 0, ns);
   // every prologue load complete (vmcnt(0); lgkmcnt and expcnt at their
   // maximum, gfx9 encoding): the loop's waits then depend on its body only
   __builtin_amdgcn_s_waitcnt(0x0F70);
 
-  for (int tl = tb; tl < te; tl += G) {
-    const int t    = uniform(tl);
-    const int e0   = tile_table(ta.elem_off, t);
-    const int ncol = tile_table(ta.num_colors, t);
+  for (int il = ib; il < ie; il += G) {
+    const int     i    = uniform(il);
+    const TileCtx c    = tile_ctx(work, i);
+    const TileCtx cn   = tile_ctx(work, next(i));
+    const int     t    = c.t;
+    const int     e0   = tile_table(ta.elem_off, t);
+    const int     ncol = tile_table(ta.num_colors, t);
     __syncthreads();  // tile t staged
 
-    // ---- prefetch (clamped to this workgroup's last tile): the next tile's
-    // nodes and elements, the node ids of the one after
+    // ---- prefetch (clamped to this workgroup's last item): the next
+    // item's nodes and elements, the node ids of the one after, and (in
+    // ensembles) the next item's sample's condition entries
 #pragma unroll
     for (int k = 0; k < 2; k++) {
-      load3(X, ids_next.gid[k], xr[k]);
+      load3(X, ids_next.gid[k] + cn.x_delta, xr[k]);
       load3(na.u, ids_next.gid[k], ur[k]);
     }
-    const ElemData el_next   = load_elem<kThreads>(ta, next(t));
-    const SlotIds  ids_after = load_ids<kThreads>(ta, next(next(t)));
+    const ElemData el_next   = load_elem<kThreads>(ta, cn.t);
+    const SlotIds  ids_after = load_ids<kThreads>(ta, tile_ctx(work, next(next(i))));
+    uint64_t       bc_next   = 0;
+    if (kEnsemble)
+      bc_next = reinterpret_cast<const uint64_t*>(work_bc(work) + cn.bc_base)[min(tid, kTileBcWords - 1)];
 
     // ---- element arithmetic (every thread; see load_elem)
     const int ntl = tile_table(ta.elem_off, t + 1) - e0;  // elements of tile t
@@ -295,7 +378,8 @@ __global__ __launch_bounds__(kThreads, 2) void k_element_tiled(const BlockTable<
           Xe[j][c] = sxu[lc[j]][c];
           ue[j][c] = sxu[lc[j]][3 + c];
         }
-      const int b = uniform(block_of(bt, (int64_t)e0));  // tiles never straddle blocks
+      float Kt, Gt;
+      tile_moduli(work, c, e0, Kt, Gt);
       // tile-blocked F and sigma: tile t owns [72 e0, 72 (e0 + n)) and
       // [48 e0, 48 (e0 + n)), in rows (component c, point pair p) of 2n
       // values, element l's points 2p and 2p+1 side by side, so each tile
@@ -320,7 +404,7 @@ __global__ __launch_bounds__(kThreads, 2) void k_element_tiled(const BlockTable<
 #pragma unroll
         for (int c = 0; c < 6; c++) store_pair<kAux>(Se[c], So[c], rS, loff, 8 * (c * 4 + p) * ntl);
       };
-      if (!mixed::element(Xe, ue, bt.K[b], bt.G[b], store, f)) bad = 1;
+      if (!mixed::element(Xe, ue, Kt, Gt, store, f)) bad = 1;
     }
 
     const bool gather = tile_table(ta.gather, t) != 0;  // uniform
@@ -366,7 +450,8 @@ __global__ __launch_bounds__(kThreads, 2) void k_element_tiled(const BlockTable<
     }
 
     // ---- interior nodes: complete step; others: this tile's partial
-    double rxs = 0.0;
+    const StepArgs sa  = tile_step(work, c);
+    double         rxs = 0.0;
 #pragma unroll
     for (int k = 0; k < 2; k++) {
       const int ln = tid + k * kThreads;
@@ -375,18 +460,18 @@ __global__ __launch_bounds__(kThreads, 2) void k_element_tiled(const BlockTable<
       if (gather) {
         // the slot's contributions in ascending corner order, from 0.0
         const uint4    iv   = *reinterpret_cast<const uint4*>(as.g.inv[ln]);
-        const unsigned w[4] = {iv.x, iv.y, iv.z, iv.w};
+        const unsigned wd[4] = {iv.x, iv.y, iv.z, iv.w};
         fn[0] = fn[1] = fn[2] = 0.0;
 #pragma unroll
         for (int j = 0; j < 8; j++) {
-          const unsigned e  = (w[j >> 1] >> (16 * (j & 1))) & 0xffffu;
+          const unsigned e  = (wd[j >> 1] >> (16 * (j & 1))) & 0xffffu;
           const int      ec = e == kNoElement ? kThreads : (int)e;
 #pragma unroll
-          for (int c = 0; c < 3; c++) fn[c] += (double)as.g.ef[3 * j + c][ec];
+          for (int cc = 0; cc < 3; cc++) fn[cc] += (double)as.g.ef[3 * j + cc][ec];
         }
       } else {
 #pragma unroll
-        for (int c = 0; c < 3; c++) fn[c] = as.acc[ln][c];
+        for (int cc = 0; cc < 3; cc++) fn[cc] = as.acc[ln][cc];
       }
       if (ids.dst[k] == kInterior) {
         if (rx.coord >= 0 && (ns.flg[k] & kReactionFlag)) rxs += fn[rx.coord];
@@ -401,20 +486,66 @@ __global__ __launch_bounds__(kThreads, 2) void k_element_tiled(const BlockTable<
     }
     if (rx.coord >= 0) {  // uniform across the grid
       const double s = block_sum<kThreads>(rxs, red);
-      if (threadIdx.x == 0) rx.partials[t] = s;
+      if (threadIdx.x == 0) rx.partials[tile_rx_slot(work, c, i)] = s;
     }
     __syncthreads();  // tile t's LDS reads done
 
-    // ---- stage the next tile
+    // ---- stage the next item
     ids      = ids_next;
     ids_next = ids_after;
     cur      = el_next;
-    if (t + G < te) stage<kThreads>(ids, xr, ur, na, sa, sxu, as, tile_table(ta.gather, t + G) != 0, ns);
+    if (i + G < ie) {
+      if (kEnsemble && tid < kTileBcWords) reinterpret_cast<uint64_t*>(sbc)[tid] = bc_next;
+      stage<kThreads>(ids, cn.x_delta, xr, ur, na, update_v, sxu, as, tile_table(ta.gather, cn.t) != 0, ns);
+    }
   }
   if (tid == 0 && bad) atomicOr(status, 1);
 }
 
 constexpr int kBndThreads = kBoundaryNodesPerGroup;
+
+// Boundary node bi (partials [p0, p1)) at row i, flags at row xi: sums its
+// partials in ascending tile order from 0.0 in FP64, then the node update.
+// Returns its reaction contribution.
+__device__ __forceinline__ double boundary_node(const TileArgs& ta, const NodeArrays& na, const BcEntry* table,
+                                                int64_t i, int64_t xi, int64_t p0, int64_t p1, const StepArgs& sa,
+                                                int rx_coord)
+{
+  // the node's own state is loaded first, so its latency overlaps the
+  // dependent partial-sum loads
+  const int flags = na.flags[xi];
+  double    u[3], v[3], m = 1.0;
+  if (sa.update_v) {
+    load3(na.u, i, u);
+    load3(na.v, i, v);
+    m = na.mass[i];
+  }
+  // up to 8 partials are loaded at once, then added in ascending order
+  // (the same order as one at a time); more than 8 continue one by one
+  constexpr int kBatch = 8;
+  float         pv[kBatch][3];
+#pragma unroll
+  for (int k = 0; k < kBatch; k++)
+    if (p0 + k < p1)
+#pragma unroll
+      for (int c = 0; c < 3; c++) pv[k][c] = ta.partials[3 * (p0 + k) + c];
+  double f[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+  for (int k = 0; k < kBatch; k++)
+    if (p0 + k < p1) {
+      f[0] += (double)pv[k][0];
+      f[1] += (double)pv[k][1];
+      f[2] += (double)pv[k][2];
+    }
+  for (int64_t p = p0 + kBatch; p < p1; p++) {
+    f[0] += (double)ta.partials[3 * p];
+    f[1] += (double)ta.partials[3 * p + 1];
+    f[2] += (double)ta.partials[3 * p + 2];
+  }
+  const double rxs = rx_coord >= 0 && (flags & kReactionFlag) ? f[rx_coord] : 0.0;
+  node_update(na, table, i, flags, f, u, v, m, sa);
+  return rxs;
+}
 
 __global__ __launch_bounds__(kBndThreads) void k_boundary_update(TileArgs ta, NodeArrays na, StepArgs sa,
                                                                  ReactionArgs rx)
@@ -424,86 +555,111 @@ __global__ __launch_bounds__(kBndThreads) void k_boundary_update(TileArgs ta, No
   const int         bi  = (int)g * kBndThreads + threadIdx.x;
   double            rxs = 0.0;
   if (bi < ta.num_bnd) {
-    // the node's own state is loaded first, so its latency overlaps the
-    // dependent partial-sum loads
-    const int64_t i     = (int64_t)ta.bnd_base + bi;
-    const int     flags = na.flags[i];
-    double        u[3], v[3], m = 1.0;
-    if (sa.update_v) {
-      load3(na.u, i, u);
-      load3(na.v, i, v);
-      m = na.mass[i];
-    }
-    const int p0 = ta.bnd_ptr[bi], p1 = ta.bnd_ptr[bi + 1];
-    // up to 8 partials are loaded at once, then added in ascending order
-    // (the same order as one at a time); more than 8 continue one by one
-    constexpr int kBatch = 8;
-    float         pv[kBatch][3];
-#pragma unroll
-    for (int i = 0; i < kBatch; i++)
-      if (p0 + i < p1)
-#pragma unroll
-        for (int c = 0; c < 3; c++) pv[i][c] = ta.partials[3 * ((int64_t)p0 + i) + c];
-    double f[3] = {0.0, 0.0, 0.0};
-#pragma unroll
-    for (int i = 0; i < kBatch; i++)
-      if (p0 + i < p1) {
-        f[0] += (double)pv[i][0];
-        f[1] += (double)pv[i][1];
-        f[2] += (double)pv[i][2];
-      }
-    for (int64_t p = (int64_t)p0 + kBatch; p < p1; p++) {
-      f[0] += (double)ta.partials[3 * p];
-      f[1] += (double)ta.partials[3 * p + 1];
-      f[2] += (double)ta.partials[3 * p + 2];
-    }
-    if (rx.coord >= 0 && (flags & kReactionFlag)) rxs = f[rx.coord];
-    node_update(na, na.bc_table, i, flags, f, u, v, m, sa);
+    const int64_t i = (int64_t)ta.bnd_base + bi;
+    rxs = boundary_node(ta, na, na.bc_table, i, i, ta.bnd_ptr[bi], ta.bnd_ptr[bi + 1], sa, rx.coord);
   }
   if (rx.coord < 0) return;
   const double s = block_sum<kBndThreads>(rxs, red);
   if (threadIdx.x == 0) rx.partials[ta.num_tiles + g] = s;
 }
 
+// Ensemble boundary nodes: group g is nodes [first, first + count) of the
+// boundary nodes of sample groups[g].s.
+__global__ __launch_bounds__(kBndThreads) void k_ens_boundary_update(TileArgs ta, EnsNodes en,
+                                                                     const EnsRange* __restrict__ groups, EnsStep es,
+                                                                     ReactionArgs rx, int rx_base)
+{
+  __shared__ double red[kBndThreads / 64];
+  const unsigned    g   = xcd_tile(blockIdx.x, gridDim.x);
+  const EnsRange    gr  = ens_load(groups, (int64_t)g);
+  const EnsSample   sm  = ens_load(en.samples, (int64_t)gr.s);
+  const EnsMesh     ms  = ens_load(en.meshes, (int64_t)sm.mesh);
+  const StepArgs    sa  = ens_step_args(es, gr.s);
+  double            rxs = 0.0;
+  if ((int)threadIdx.x < gr.count) {
+    const int     bi  = gr.first + (int)threadIdx.x;
+    const int64_t loc = (int64_t)ms.num_interior + bi;  // mesh-local device row
+    NodeArrays    na;
+    na.num_nodes   = 0;
+    na.slot_stride = 0;
+    na.node_ptr    = nullptr;
+    na.node_ent    = nullptr;
+    na.mass        = en.mass;
+    na.u           = en.u;
+    na.v           = en.v;
+    na.a           = en.a;
+    na.f           = en.f;
+    na.flags       = en.flags;
+    na.bc_table    = en.bc;
+    na.num_bc      = 0;
+    const int64_t p0 = (int64_t)sm.partial_base + ta.bnd_ptr[ms.bnd_pos + bi];
+    const int64_t p1 = (int64_t)sm.partial_base + ta.bnd_ptr[ms.bnd_pos + bi + 1];
+    rxs = boundary_node(ta, na, en.bc + sm.bc_base, sm.node_base + loc, sm.x_base + loc, p0, p1, sa, rx.coord);
+  }
+  if (rx.coord < 0) return;
+  const double s = block_sum<kBndThreads>(rxs, red);
+  if (threadIdx.x == 0) rx.partials[rx_base + (int)g] = s;
+}
+
 }  // namespace
 
 unsigned boundary_grid(int num_bnd) { return (unsigned)((num_bnd + kBndThreads - 1) / kBndThreads); }
 
-template <int kThreads>
-void launch_tiles(unsigned grid, const BlockTable<float>& bt, const TileArgs& ta, int t_begin, int t_count,
-                  const double* X, const NodeArrays& na, float* Fip, float* sip, const StepArgs& sa, ReactionArgs rx,
-                  ReactionFold fold, int* status, hipStream_t s)
+template <int kThreads, typename Work>
+void launch_tiles(unsigned grid, const Work& work, const TileArgs& ta, const double* X, const NodeArrays& na,
+                  float* Fip, float* sip, ReactionArgs rx, int* status, hipStream_t s)
 {
   if (Fip)
-    hipLaunchKernelGGL((k_element_tiled<kThreads, true>), dim3(grid), dim3(kThreads), 0, s, bt, ta, t_begin,
-                       t_count, X, na, Fip, sip, sa, rx, fold, status);
+    hipLaunchKernelGGL((k_element_tiled<kThreads, true, Work>), dim3(grid), dim3(kThreads), 0, s, work, ta, X, na,
+                       Fip, sip, rx, status);
   else
-    hipLaunchKernelGGL((k_element_tiled<kThreads, false>), dim3(grid), dim3(kThreads), 0, s, bt, ta, t_begin,
-                       t_count, X, na, Fip, sip, sa, rx, fold, status);
+    hipLaunchKernelGGL((k_element_tiled<kThreads, false, Work>), dim3(grid), dim3(kThreads), 0, s, work, ta, X, na,
+                       Fip, sip, rx, status);
+}
+
+template <typename Work>
+void launch_work(const Work& work, int count, const TileArgs& ta, const double* X, const NodeArrays& na, float* Fip,
+                 float* sip, ReactionArgs rx, int* status, hipStream_t s)
+{
+  if (count <= 0) return;
+  const unsigned grid = (unsigned)std::min(count, std::max(ta.resident_groups, 1));
+  switch (ta.tile_elems) {
+    case 64: launch_tiles<64>(grid, work, ta, X, na, Fip, sip, rx, status, s); break;
+    case 128: launch_tiles<128>(grid, work, ta, X, na, Fip, sip, rx, status, s); break;
+    case 512: launch_tiles<512>(grid, work, ta, X, na, Fip, sip, rx, status, s); break;
+    default: launch_tiles<256>(grid, work, ta, X, na, Fip, sip, rx, status, s);
+  }
 }
 
 void launch_element_tiled(const BlockTable<float>& bt, const TileArgs& ta, int t_begin, int t_count,
                           const double* X, const NodeArrays& na, float* Fip, float* sip, const StepArgs& sa,
                           ReactionArgs rx, ReactionFold fold, int* status, hipStream_t s)
 {
-  if (t_count <= 0) return;
-  const unsigned grid = (unsigned)std::min(t_count, std::max(ta.resident_groups, 1));
-  switch (ta.tile_elems) {
-    case 64: launch_tiles<64>(grid, bt, ta, t_begin, t_count, X, na, Fip, sip, sa, rx, fold, status, s); break;
-    case 128: launch_tiles<128>(grid, bt, ta, t_begin, t_count, X, na, Fip, sip, sa, rx, fold, status, s); break;
-    case 512: launch_tiles<512>(grid, bt, ta, t_begin, t_count, X, na, Fip, sip, sa, rx, fold, status, s); break;
-    default: launch_tiles<256>(grid, bt, ta, t_begin, t_count, X, na, Fip, sip, sa, rx, fold, status, s);
-  }
+  const ModelTileWork work{bt, t_begin, t_count, sa, fold};
+  launch_work(work, t_count, ta, X, na, Fip, sip, rx, status, s);
+}
+
+void launch_ens_element_tiled(const EnsTileWork& work, const TileArgs& ta, const double* X, const NodeArrays& na,
+                              ReactionArgs rx, int* status, hipStream_t s)
+{
+  launch_work(work, work.count, ta, X, na, nullptr, nullptr, rx, status, s);
 }
 
 int element_tiled_groups_per_cu(int tile_elems)
 {
   int n = 0;
   switch (tile_elems) {
-    case 64: (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_element_tiled<64, true>, 64, 0); break;
-    case 128: (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_element_tiled<128, true>, 128, 0); break;
-    case 512: (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_element_tiled<512, true>, 512, 0); break;
-    default: (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_element_tiled<256, true>, 256, 0);
+    case 64:
+      (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_element_tiled<64, true, ModelTileWork>, 64, 0);
+      break;
+    case 128:
+      (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_element_tiled<128, true, ModelTileWork>, 128, 0);
+      break;
+    case 512:
+      (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_element_tiled<512, true, ModelTileWork>, 512, 0);
+      break;
+    default:
+      (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_element_tiled<256, true, ModelTileWork>, 256, 0);
   }
   return n;
 }
@@ -513,6 +669,14 @@ void launch_boundary_update(const TileArgs& ta, const NodeArrays& na, const Step
 {
   if (ta.num_bnd <= 0) return;
   hipLaunchKernelGGL(k_boundary_update, dim3(boundary_grid(ta.num_bnd)), dim3(kBndThreads), 0, s, ta, na, sa, rx);
+}
+
+void launch_ens_boundary_update(const TileArgs& ta, const EnsNodes& en, const EnsRange* groups, int num_groups,
+                                EnsStep es, ReactionArgs rx, int rx_base, hipStream_t s)
+{
+  if (num_groups <= 0) return;
+  hipLaunchKernelGGL(k_ens_boundary_update, dim3((unsigned)num_groups), dim3(kBndThreads), 0, s, ta, en, groups, es,
+                     rx, rx_base);
 }
 
 }  // namespace nsm
