@@ -9,12 +9,23 @@ before timing starts.
 
     python bench.py [--gpus 1] [--steps 500] [--warmup 20] [--precision mixed|fp64]
                     [--workload cube128_2blk] [--cpu-steps 40] [--no-cpu] [--uq NE,NA]
+                    [--no-fp64] [--fp64-steps 20] [--no-ensemble]
 
 Prints one JSON line.  `value` is element-steps/s of the timed K steps.
 `roofline` covers the dominant (element) kernel: its algorithmic bytes per
 launch (algorithmic_bytes below) over its average launch duration from HIP
 events on the model's stream.  `cpu_baseline` times the FP64 CPU oracle (OpenMP, deterministic
 gather) on a bounded sample of the same workload.
+
+The same line carries an `fp64` object (the plain FP64 path on the same
+workload, --fp64-steps timed steps) and an `ensemble` object: the batched
+ensemble of 256 samples on the 32^3 cube plus 64 on cubes with edge counts
+drawn from {16, 24, 32, 40, 48} (per-sample E, nu, rho lognormal around
+200 GPa / 0.3 / 7800, top velocity uniform in [0.5, 2] m/s, 400 steps),
+replayed from a captured HIP graph: samples/s of whole runs, the mean-QoI
+error of the mixed ensemble against the FP64 ensemble, and plain
+one-at-a-time solves of the model path (create, lumped mass and dt, 400
+steps, QoI) timed on two samples per edge count.
 
 --uq NE,NA adds an off-nominal ensemble (NE exact samples with the bulk and
 shear moduli of every block scaled by 1 + 0.2 x_k, x_k evenly spaced in
@@ -166,6 +177,125 @@ def cpu_baseline(prob, steps, runs=5):
     }
 
 
+def fp64_leg(N, workload, steps, warmup):
+    """The plain FP64 path on the same workload: a few timed steps (the
+    element kernel is VALU-bound, so a short run gives its rate)."""
+    import torch
+
+    prob = N.workload(workload).build()
+    model = prob.make_model("fp64")
+    integ = prob.make_integrator(model, num_steps=warmup + steps)
+    integ.Advance(warmup)
+    model.EnableKernelTiming(True)
+    model.Synchronize()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    integ.Advance(steps)
+    model.Synchronize()
+    elapsed = time.perf_counter() - t0
+    el_ms, _ = model.KernelTime("element")
+    nd_ms, _ = model.KernelTime("node")
+    model.EnableKernelTiming(False)
+    per_launch, step_bytes = algorithmic_bytes(prob, 8, model.AssemblyInfo())
+    ms = elapsed * 1e3 / steps
+    model.close()
+    return {
+        "steps": steps, "warmup": warmup, "ms_per_step": ms, "value": prob.num_elem * steps / elapsed,
+        "unit": "element-steps/s", "element_ms_avg": el_ms, "node_ms_avg": nd_ms,
+        "element_alg_bytes_per_launch": float(np.mean(per_launch)),
+        "element_frac": float(np.mean(per_launch)) / (el_ms * 1e-3) / 1e9 / HBM_PEAK_GBS if el_ms > 0 else 0.0,
+        "step_frac": step_bytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+    }
+
+
+def ensemble_leg(N, num_steps=400, runs=3, plain_per_size=2):
+    """Batched ensemble of 256 samples on the 32^3 cube and 64 on uneven
+    cubes (edge counts from {16, 24, 32, 40, 48}), num_steps steps each:
+    samples/s of whole runs (graph replays, mixed precision), the mean-QoI
+    error against the same ensemble in FP64, and plain one-at-a-time solves of
+    the model path timed on a stratified subset (plain_per_size samples per
+    edge count, scaled by the counts)."""
+    import torch
+
+    from nimblesm_amd.ensemble import EnsembleProblem, config2_draws
+
+    d = config2_draws()
+    out = {"workload": "256 x 32^3 + 64 x n^3, n uniform in {16,24,32,40,48}", "samples": len(d),
+           "steps": num_steps, "elements_per_step": int(np.sum(np.asarray(d.n) ** 3))}
+    res = {}
+    for precision in ("mixed", "fp64"):
+        t_setup = time.perf_counter()
+        prob = EnsembleProblem(d, num_steps=num_steps, precision=precision).build()
+        t_setup = time.perf_counter() - t_setup
+        q = prob.run(use_graph=True)  # capture + first replay
+        torch.cuda.synchronize()
+        times = []
+        for _ in range(runs):
+            t0 = time.perf_counter()
+            q = prob.run(use_graph=True)  # replay + QoI readout
+            times.append(time.perf_counter() - t0)
+        info = prob.ens.Info()
+        # per-kernel durations from one eager run with HIP events
+        prob.ens.EnableKernelTiming(True)
+        prob.run(use_graph=False)
+        el_ms, _ = prob.ens.KernelTime("element")
+        nd_ms, _ = prob.ens.KernelTime("node")
+        prob.ens.EnableKernelTiming(False)
+        prob.close()
+        run_s = float(np.median(times))
+        res[precision] = q
+        out[precision] = {
+            "run_s": run_s, "samples_per_s": len(d) / run_s,
+            "element_steps_per_s": out["elements_per_step"] * num_steps / run_s,
+            "element_ms_avg": el_ms, "node_ms_avg": nd_ms, "tile_elems": info["tile_elems"],
+            "work_items_per_step": info["work_items"], "setup_s": t_setup, "graph": True,
+        }
+    ref = res["fp64"]
+    err = np.linalg.norm(res["mixed"].mean(0) - ref.mean(0)) / np.linalg.norm(ref.mean(0))
+    per = np.linalg.norm(res["mixed"] - ref, axis=1) / np.linalg.norm(ref, axis=1)
+    out["qoi_mean_rel_err_mixed_vs_fp64"] = float(err)
+    out["qoi_worst_sample_rel_err_mixed_vs_fp64"] = float(per.max())
+
+    # plain one-at-a-time solves: model creation (tiles), lumped mass and dt,
+    # num_steps steps, QoI sampling -- and the steps alone
+    plain = {}
+    for precision in ("mixed", "fp64"):
+        solve_total = steps_total = 0.0
+        detail = {}
+        for n in sorted(set(int(x) for x in d.n)):
+            idx = np.nonzero(d.n == n)[0][:plain_per_size]
+            ts, tt = [], []
+            for i in idx:
+                torch.cuda.synchronize()
+                t0 = time.perf_counter()
+                prob1 = N.CubeProblem(n=n, materials=[(d.E[i], d.nu[i], d.rho[i])], num_steps=num_steps,
+                                      v_top=float(d.v_top[i]), name="plain")
+                m = prob1.make_model(precision)
+                integ = prob1.make_integrator(m)
+                m.Synchronize()
+                t1 = time.perf_counter()
+                integ.Integrate()
+                m.Synchronize()
+                t2 = time.perf_counter()
+                m.SampleReactionHistory(num_steps, 64)
+                t3 = time.perf_counter()
+                m.close()
+                ts.append(t3 - t0)
+                tt.append(t2 - t1)
+            cnt = int(np.sum(d.n == n))
+            solve_total += cnt * float(np.mean(ts))
+            steps_total += cnt * float(np.mean(tt))
+            detail[str(n)] = {"count": cnt, "solve_ms": 1e3 * float(np.mean(ts)), "steps_ms": 1e3 * float(np.mean(tt))}
+        plain[precision] = {"samples_per_s": len(d) / solve_total, "steps_only_samples_per_s": len(d) / steps_total,
+                            "per_edge_count": detail, "timed_per_edge_count": plain_per_size}
+    out["plain"] = plain
+    out["speedup_vs_plain"] = out["mixed"]["samples_per_s"] / plain["mixed"]["samples_per_s"]
+    out["speedup_vs_plain_steps_only"] = out["mixed"]["samples_per_s"] / plain["mixed"]["steps_only_samples_per_s"]
+    out["fp64_speedup_vs_plain_fp64_steps_only"] = (out["fp64"]["samples_per_s"] /
+                                                    plain["fp64"]["steps_only_samples_per_s"])
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -176,6 +306,9 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=40)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--uq", default="", metavar="NE,NA", help="add NE exact and NA approximate UQ samples")
+    ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 leg")
+    ap.add_argument("--fp64-steps", type=int, default=20)
+    ap.add_argument("--no-ensemble", action="store_true", help="skip the batched-ensemble leg")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run 5 device copies of MIB MiB (calibrates rocprofv3 FETCH_SIZE / WRITE_SIZE)")
     args = ap.parse_args()
@@ -317,6 +450,11 @@ def main():
             "closure_achieved_GBs": cb / (cl_ms * 1e-3) / 1e9 if cl_ms > 0 else 0.0,
             "closure_frac": cb / (cl_ms * 1e-3) / 1e9 / HBM_PEAK_GBS if cl_ms > 0 else 0.0,
         }
+    model.close()
+    if not args.no_fp64 and args.precision == "mixed" and not args.uq:
+        out["fp64"] = fp64_leg(N, args.workload, args.fp64_steps, 5)
+    if not args.no_ensemble and not args.uq:
+        out["ensemble"] = ensemble_leg(N)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(prob, args.cpu_steps)
     print(json.dumps(out), flush=True)
