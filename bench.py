@@ -27,6 +27,12 @@ error of the mixed ensemble against the FP64 ensemble, and plain
 one-at-a-time solves of the model path (create, lumped mass and dt, 400
 steps, QoI) timed on two samples per edge count.
 
+A `surrogate` object covers the learned control variate on the 24^3 cube:
+512 plain FP64 solves (batched) train the 4-256-256-256-64 network on the
+device; 8192 draws are estimated with 5% plain solves for bias correction;
+samples/s of the whole estimator, its error against plain solves of all
+draws, and plain one-at-a-time solves timed beside it.
+
 --uq NE,NA adds an off-nominal ensemble (NE exact samples with the bulk and
 shear moduli of every block scaled by 1 + 0.2 x_k, x_k evenly spaced in
 [-1, 1], and NA approximate samples with closure coefficients interpolating
@@ -296,6 +302,103 @@ def ensemble_leg(N, num_steps=400, runs=3, plain_per_size=2):
     return out
 
 
+def surrogate_leg(N, num_steps=400, n_train=512, n_eval=8192, fraction=0.05, iterations=3000):
+    """Learned control variate on the 24^3 cube: n_train plain FP64 solves
+    (batched) train the network on the device; n_eval draws are estimated
+    with `fraction` of them solved plainly (batched, mixed precision) for
+    bias correction and error judging.  samples/s = n_eval / wall time of the
+    whole estimator (training solves, training, inference, correction
+    solves); against plain one-at-a-time solves of the model path timed on 4
+    draws.  The error is against the mean of plain solves of all n_eval draws
+    (batched, untimed)."""
+    import torch
+
+    from nimblesm_amd.ensemble import EnsembleProblem, draw_parameters
+    from nimblesm_amd.surrogate import LinearSurrogate, MLPSurrogate, control_variate
+
+    train = draw_parameters(n_train, seed=101, n=24)
+    ev = draw_parameters(n_eval, seed=202, n=24)
+
+    def solve(d, precision):
+        prob = EnsembleProblem(d, num_steps=num_steps, precision=precision).build()
+        q = prob.run()
+        prob.close()
+        return q
+
+    # one-time autotuning of the tile size for this shape (cached for the process)
+    t0 = time.perf_counter()
+    solve(ev.subset(np.arange(8)), "mixed")
+    autotune_s = time.perf_counter() - t0
+    truth_all = solve(ev, "mixed")
+    truth = truth_all.mean(axis=0)
+    torch.cuda.synchronize()
+
+    phases = {}
+    t = time.perf_counter()
+    y_train = solve(train, "fp64")
+    phases["training_solves_s"] = time.perf_counter() - t
+    t = time.perf_counter()
+    mlp = MLPSurrogate(seed=0).fit(train, y_train, iterations=iterations)
+    phases["training_s"] = time.perf_counter() - t
+    t = time.perf_counter()
+    pred = mlp.predict(ev)
+    phases["inference_s"] = time.perf_counter() - t
+    t = time.perf_counter()
+    est = control_variate(mlp, ev, lambda d: solve(d, "mixed"), fraction=fraction, seed=7, pred=pred)
+    phases["correction_solves_s"] = time.perf_counter() - t
+    total = sum(phases.values())
+    err = float(np.linalg.norm(est.mean - truth) / np.linalg.norm(truth))
+
+    # equal cost comparisons: the same number of plain solves
+    rng = np.random.default_rng(8)
+    n_plain = n_train + est.num_plain
+    mc = rng.choice(n_eval, size=n_plain, replace=False)
+    err_mc = float(np.linalg.norm(truth_all[mc].mean(0) - truth) / np.linalg.norm(truth))
+    idx = np.sort(np.random.default_rng(7).choice(n_eval, size=est.num_plain, replace=False))
+    lin = LinearSurrogate().fit(train, y_train)
+    est_lin = control_variate(lin, ev, lambda d: truth_all[idx], plain_idx=idx)
+    err_lin = float(np.linalg.norm(est_lin.mean - truth) / np.linalg.norm(truth))
+
+    # plain one-at-a-time solves of the model path (24^3)
+    plain = {}
+    for precision in ("mixed", "fp64"):
+        ts, tt = [], []
+        for i in range(4):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            prob1 = N.CubeProblem(n=24, materials=[(ev.E[i], ev.nu[i], ev.rho[i])], num_steps=num_steps,
+                                  v_top=float(ev.v_top[i]), name="plain")
+            m = prob1.make_model(precision)
+            integ = prob1.make_integrator(m)
+            m.Synchronize()
+            t1 = time.perf_counter()
+            integ.Integrate()
+            m.Synchronize()
+            t2 = time.perf_counter()
+            m.SampleReactionHistory(num_steps, 64)
+            ts.append(time.perf_counter() - t0)
+            tt.append(t2 - t1)
+            m.close()
+        plain[precision] = {"solve_ms": 1e3 * float(np.median(ts)), "steps_ms": 1e3 * float(np.median(tt)),
+                            "samples_per_s": 1.0 / float(np.median(ts)),
+                            "steps_only_samples_per_s": 1.0 / float(np.median(tt))}
+    sps = n_eval / total
+    mlp.close()
+    return {
+        "workload": f"24^3 cube, {n_train} FP64 training solves, {n_eval} draws, {fraction:.0%} plain solves",
+        "network": "4-256-256-256-64 GELU, BF16 weights/activations, FP32 accumulation (MFMA 16x16x32)",
+        "iterations": iterations, "final_loss": float(mlp.loss[-1]), "initial_loss": float(mlp.loss[0]),
+        "samples_per_s": sps, "wall_s": total, "phases": phases, "autotune_s": autotune_s,
+        "qoi_mean_rel_err": err, "judged_rel_err": est.est_rel_err, "heldout_rel_err": est.heldout_rel_err,
+        "fallback": est.fallback, "plain_solves": n_plain,
+        "equal_cost_plain_mc_rel_err": err_mc, "equal_cost_linear_cv_rel_err": err_lin,
+        "plain_one_at_a_time": plain,
+        "speedup_vs_plain": sps / plain["mixed"]["samples_per_s"],
+        "speedup_vs_plain_steps_only": sps / plain["mixed"]["steps_only_samples_per_s"],
+        "speedup_vs_plain_fp64_steps_only": sps / plain["fp64"]["steps_only_samples_per_s"],
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -309,6 +412,7 @@ def main():
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 leg")
     ap.add_argument("--fp64-steps", type=int, default=20)
     ap.add_argument("--no-ensemble", action="store_true", help="skip the batched-ensemble leg")
+    ap.add_argument("--no-surrogate", action="store_true", help="skip the learned-control-variate leg")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run 5 device copies of MIB MiB (calibrates rocprofv3 FETCH_SIZE / WRITE_SIZE)")
     args = ap.parse_args()
@@ -455,6 +559,8 @@ def main():
         out["fp64"] = fp64_leg(N, args.workload, args.fp64_steps, 5)
     if not args.no_ensemble and not args.uq:
         out["ensemble"] = ensemble_leg(N)
+    if not args.no_surrogate and not args.uq:
+        out["surrogate"] = surrogate_leg(N)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(prob, args.cpu_steps)
     print(json.dumps(out), flush=True)

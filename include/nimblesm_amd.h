@@ -251,6 +251,14 @@ int nsm_model_get_element_order(nsm_model* model, int64_t* out);
  * src/nimble_kokkos_model_data.cc:1259-1280). */
 int nsm_model_assembly_info(nsm_model* model, int* tiled, int64_t* num_tiles, int64_t* interior_nodes,
                             int64_t* boundary_nodes, int64_t* gather_tiles);
+/* Elements per element tile (= threads per workgroup of the tiled kernel) of
+ * a tiled mixed model, 0 otherwise.  Chosen at creation: NSM_TILE_ELEMS (64,
+ * 128, 256, 512) when set, 512 when that gives every CU two tiles, else by
+ * the autotuner from a few timed steps with each size (cached per mesh shape
+ * for the process, and in the file NSM_AUTOTUNE_CACHE names).  No reference
+ * counterpart (the reference's Kokkos policies are fixed,
+ * src/nimble_kokkos_block_material_interface.cc:76-80). */
+int nsm_model_tile_elems(nsm_model* model, int* tile_elems);
 
 int nsm_model_synchronize(nsm_model* model);
 /* Launch-duration statistics gathered by HIP events around every launch of
@@ -383,8 +391,44 @@ int nsm_ensemble_get_field(nsm_ensemble* ens, int sample, int field, double* out
  * (mixed; element chunks in FP64), the tile size in use, device bytes. */
 int nsm_ensemble_info(nsm_ensemble* ens, int64_t* num_samples, int64_t* num_nodes, int64_t* num_elem,
                       int64_t* work_items, int* tile_elems, int64_t* device_bytes);
+/* What the autotuner measured at setup (tile_elems 0 at create): count
+ * candidates, their tile sizes and seconds per step (negative: unusable);
+ * count 0 when the choice came from the cache or was given. */
+int nsm_ensemble_autotune_timings(nsm_ensemble* ens, int capacity, int* count, int* tile_elems,
+                                  double* seconds_per_step);
 int nsm_ensemble_enable_kernel_timing(nsm_ensemble* ens, int enable);
 int nsm_ensemble_kernel_time(nsm_ensemble* ens, int kernel, double* avg_ms, int64_t* launches);
+
+/* ---------------------------------------------------------------------------
+ * Surrogate network of the ensemble estimator (no counterpart in the
+ * reference): fully connected widths[0] -> widths[1] -> ... ->
+ * widths[num_layers], GELU (error-function form) after every layer but the
+ * last.  Weights and activations are BF16, products accumulate in FP32 on the
+ * matrix cores; FP32 master parameters are trained by Adam on the mean
+ * squared error.  Parameters are ordered W_1 [d_1][d_0] (row-major), b_1
+ * [d_1], W_2, b_2, ...  All arrays below are host arrays of FP32 rows.
+ * ------------------------------------------------------------------------- */
+typedef struct nsm_mlp nsm_mlp;
+
+/* Weights uniform in +-sqrt(6 / (fan_in + fan_out)) from SplitMix64(seed),
+ * zero biases. */
+int nsm_mlp_create(int num_layers, const int* widths, uint64_t seed, nsm_mlp** out);
+int nsm_mlp_destroy(nsm_mlp* net);
+int nsm_mlp_num_parameters(nsm_mlp* net, int64_t* count);
+int nsm_mlp_get_parameters(nsm_mlp* net, float* out);
+/* Replaces the parameters and resets the optimizer state. */
+int nsm_mlp_set_parameters(nsm_mlp* net, const float* in);
+/* y [batch][d_L] from x [batch][d_0]. */
+int nsm_mlp_forward(nsm_mlp* net, int64_t batch, const float* x, float* y);
+/* Mean over batch and outputs of (y - target)^2 and its gradient with
+ * respect to every parameter (in parameter order). */
+int nsm_mlp_loss_gradients(nsm_mlp* net, int64_t batch, const float* x, const float* target, double* loss,
+                           float* grad);
+/* `iterations` full-batch Adam steps (beta 0.9, 0.999, eps 1e-8) with the
+ * learning rate decaying from lr to lr_final on a cosine; the loss before
+ * each step goes to loss_history [iterations] (may be NULL). */
+int nsm_mlp_train(nsm_mlp* net, int64_t batch, const float* x, const float* target, int iterations, double lr,
+                  double lr_final, double* loss_history);
 
 /* ---------------------------------------------------------------------------
  * Synthetic input: n^3 HEX8 unit cube, node (i,j,k) -> i+(n+1)(j+(n+1)k),

@@ -69,6 +69,7 @@ SIGNATURES = {
     "nsm_model_get_node_order": (_I, [_P, _P]),
     "nsm_model_assembly_info": (_I, [_P, C.POINTER(_I), C.POINTER(_I64), C.POINTER(_I64), C.POINTER(_I64),
                                        C.POINTER(_I64)]),
+    "nsm_model_tile_elems": (_I, [_P, C.POINTER(_I)]),
     "nsm_model_info": (_I, [_P, C.POINTER(_I64), C.POINTER(_I64), C.POINTER(_I64), C.POINTER(_I64)]),
     "nsm_ensemble_create": (_I, [_I, _I, C.POINTER(_P)]),
     "nsm_ensemble_destroy": (_I, [_P]),
@@ -85,8 +86,17 @@ SIGNATURES = {
     "nsm_ensemble_get_field": (_I, [_P, _I, _I, _P]),
     "nsm_ensemble_info": (_I, [_P, C.POINTER(_I64), C.POINTER(_I64), C.POINTER(_I64), C.POINTER(_I64),
                                C.POINTER(_I), C.POINTER(_I64)]),
+    "nsm_ensemble_autotune_timings": (_I, [_P, _I, C.POINTER(_I), _P, _P]),
     "nsm_ensemble_enable_kernel_timing": (_I, [_P, _I]),
     "nsm_ensemble_kernel_time": (_I, [_P, _I, C.POINTER(_D), C.POINTER(_I64)]),
+    "nsm_mlp_create": (_I, [_I, _P, C.c_uint64, C.POINTER(_P)]),
+    "nsm_mlp_destroy": (_I, [_P]),
+    "nsm_mlp_num_parameters": (_I, [_P, C.POINTER(_I64)]),
+    "nsm_mlp_get_parameters": (_I, [_P, _P]),
+    "nsm_mlp_set_parameters": (_I, [_P, _P]),
+    "nsm_mlp_forward": (_I, [_P, _I64, _P, _P]),
+    "nsm_mlp_loss_gradients": (_I, [_P, _I64, _P, _P, C.POINTER(_D), _P]),
+    "nsm_mlp_train": (_I, [_P, _I64, _P, _P, _I, _D, _D, _P]),
     "nsm_jittered_cube": (_I, [_I, C.c_uint64, _D, _P, _P]),
 }
 

@@ -39,6 +39,9 @@ SOURCES = {
     "nsm_tiles.cpp": ["-ffp-contract=off", "-x", "hip"],
     "nsm_model.cpp": ["-ffp-contract=off", "-x", "hip"],
     "nsm_ensemble.cpp": ["-ffp-contract=off", "-x", "hip"],
+    "nsm_autotune.cpp": [],
+    # surrogate network: BF16 matrix-core GEMMs, FP32 epilogues and Adam
+    "nsm_mlp.hip": ["-ffp-contract=fast-honor-pragmas"],
 }
 
 

@@ -12,6 +12,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -23,6 +24,7 @@
 #include <vector>
 
 #include "../../include/nimblesm_amd.h"
+#include "nsm_autotune.h"
 #include "nsm_ensemble.h"
 #include "nsm_kernels.h"
 #include "nsm_tiled.h"
@@ -187,6 +189,7 @@ struct nsm_ensemble {
   double*         d_qoi = nullptr;
   int64_t         qoi_cap = 0;
   std::map<std::pair<int64_t, int64_t>, hipGraphExec_t> graphs;
+  std::vector<std::pair<int, double>>                   tune_timings;  // autotuner: (tile size, s per step)
 
   bool                                timing = false;
   std::vector<EventPair>              event_pool;
@@ -884,6 +887,68 @@ int nsm_ensemble_set_samples(nsm_ensemble* es, int first_sample, int num_samples
   return st ? st : compute_mass_and_dt(es);
 }
 
+}  // extern "C"
+
+namespace {
+
+constexpr int kTuneSamplesPerMesh = 256;
+
+// Element-tile size of a mixed ensemble chosen by the autotuner from timed
+// steps of the same meshes with at most kTuneSamplesPerMesh samples each
+// (enough to fill the device), cached per ensemble shape.
+int ensemble_tile_elems(nsm_ensemble* es, std::vector<std::pair<int, double>>* timings)
+{
+  std::string key = "ensemble:mixed";
+  for (const auto& m : es->meshes) {
+    char b[96];
+    snprintf(b, sizeof(b), ":N%lld,E%lld,B%d,S%d", (long long)m.N, (long long)m.E, m.nb,
+             (int)std::min<size_t>(m.samples.size(), kTuneSamplesPerMesh));
+    key += b;
+  }
+  auto time_of = [&](int c) -> double {
+    nsm_ensemble* tmp = nullptr;
+    if (nsm_ensemble_create(NSM_MIXED, c, &tmp)) return -1.0;
+    for (const auto& m : es->meshes) {
+      HostMesh d;
+      d.N        = m.N;
+      d.E        = m.E;
+      d.nb       = m.nb;
+      d.blk_off  = m.blk_off;
+      d.X        = m.X;
+      d.conn     = m.conn;
+      d.bcs      = m.bcs;
+      d.rx_nodes = m.rx_nodes;
+      d.rx_coord = m.rx_coord;
+      tmp->meshes.push_back(std::move(d));
+    }
+    for (size_t mi = 0; mi < es->meshes.size(); mi++) {
+      const auto& ids = es->meshes[mi].samples;
+      for (size_t k = 0; k < std::min<size_t>(ids.size(), kTuneSamplesPerMesh); k++) {
+        tmp->meshes[mi].samples.push_back((int)tmp->samples.size());
+        tmp->samples.push_back(es->samples[(size_t)ids[k]]);
+      }
+    }
+    double sec = -1.0;
+    if (!nsm_ensemble_setup(tmp)) {
+      std::vector<double> ft(tmp->critical_dt.size());
+      for (size_t s = 0; s < ft.size(); s++) ft[s] = 6.0 * 0.5 * tmp->critical_dt[s];
+      if (!nsm_ensemble_run(tmp, 2, 0, ft.data(), 0)) {
+        const auto t0 = std::chrono::steady_clock::now();
+        if (!nsm_ensemble_run(tmp, 6, 0, ft.data(), 0))
+          sec = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count() / 6.0;
+      }
+    }
+    nsm_ensemble_destroy(tmp);
+    return sec;
+  };
+  const int c = nsm::autotune_tiles(key, time_of, timings);
+  return c ? c : 512;
+}
+
+}  // namespace
+
+extern "C" {
+
 int nsm_ensemble_setup(nsm_ensemble* es)
 {
   if (!es) return fail(NSM_ERR_ARG, "null ensemble");
@@ -898,7 +963,9 @@ int nsm_ensemble_setup(nsm_ensemble* es)
     }
   es->rx_coord = rx_coord;
   ENS_HIP(hipSetDevice(es->device));
-  es->tile_elems = es->tile_request ? es->tile_request : 512;
+  es->tile_elems = es->tile_request                ? es->tile_request
+                   : es->precision == NSM_MIXED ? ensemble_tile_elems(es, &es->tune_timings)
+                                                  : 0;
   int st = NSM_OK;
   for (auto& m : es->meshes)
     if ((st = prepare_mesh(es, m))) return st;
@@ -1058,6 +1125,18 @@ int nsm_ensemble_info(nsm_ensemble* es, int64_t* num_samples, int64_t* num_nodes
   if (work_items) *work_items = es->precision == NSM_MIXED ? es->num_tile_work : es->num_elem_chunks;
   if (tile_elems) *tile_elems = es->precision == NSM_MIXED ? es->tile_elems : 0;
   if (device_bytes) *device_bytes = (int64_t)es->device_bytes;
+  return NSM_OK;
+}
+
+int nsm_ensemble_autotune_timings(nsm_ensemble* es, int capacity, int* count, int* tile_elems,
+                                  double* seconds_per_step)
+{
+  if (!es || !count) return fail(NSM_ERR_ARG, "null argument");
+  *count = (int)es->tune_timings.size();
+  for (int k = 0; k < std::min(capacity, *count); k++) {
+    if (tile_elems) tile_elems[k] = es->tune_timings[(size_t)k].first;
+    if (seconds_per_step) seconds_per_step[k] = es->tune_timings[(size_t)k].second;
+  }
   return NSM_OK;
 }
 

@@ -11,6 +11,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -22,6 +23,7 @@
 #include <vector>
 
 #include "../../include/nimblesm_amd.h"
+#include "nsm_autotune.h"
 #include "nsm_kernels.h"
 #include "nsm_node_step.h"
 #include "nsm_tiled.h"
@@ -825,9 +827,78 @@ int nsm_neohookean_get_stress(int64_t num_points, double bulk_modulus, double sh
 }
 
 // ------------------------------------------------------------------- model
+}  // extern "C"
+
+namespace {
+
+int model_create(int64_t num_nodes, const double* reference_coordinates, int num_blocks, const int* block_ids,
+                 const int64_t* block_num_elem, const int* conn, const double* density, const double* bulk_modulus,
+                 const double* shear_modulus, int precision, int tile_override, nsm_model** out);
+
+// Element-tile size of a mixed model: NSM_TILE_ELEMS when it names one;
+// 512 when that already gives every CU two tiles or more; otherwise the
+// autotuner's choice from a few timed steps of a model built with each
+// candidate (cached per mesh shape).
+int model_tile_elems(int64_t num_nodes, const double* X, int num_blocks, const int* block_ids,
+                     const int64_t* block_num_elem, const int* conn, const double* density, const double* K,
+                     const double* G, int64_t E)
+{
+  const char* te = std::getenv("NSM_TILE_ELEMS");
+  if (te && std::strcmp(te, "auto") != 0) {
+    for (int c : nsm::kTileCandidates)
+      if (std::atoi(te) == c) return c;
+    return nsm::kTileElems;
+  }
+  int dev = 0, cus = 256;
+  if (hipGetDevice(&dev) == hipSuccess)
+    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+  if (E >= 2 * (int64_t)cus * nsm::kTileElems) return nsm::kTileElems;
+  char key[160];
+  snprintf(key, sizeof(key), "model:mixed:N%lld:E%lld:B%d", (long long)num_nodes, (long long)E, num_blocks);
+  auto time_of = [&](int c) -> double {
+    nsm_model* tmp = nullptr;
+    if (model_create(num_nodes, X, num_blocks, block_ids, block_num_elem, conn, density, K, G, NSM_MIXED, c, &tmp))
+      return -1.0;
+    double dt = 0.0, t = 0.0, sec = -1.0;
+    if (tmp->tiled && !nsm_model_compute_lumped_mass(tmp) && !nsm_model_get_critical_time_step(tmp, &dt) &&
+        !nsm_model_explicit_steps(tmp, 0, 2, 8, 0, 0.5 * dt, &t)) {
+      const auto t0 = std::chrono::steady_clock::now();
+      if (!nsm_model_explicit_steps(tmp, 2, 8, 8, 0, 0.5 * dt, &t))
+        sec = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count() / 6.0;
+    }
+    nsm_model_destroy(tmp);
+    return sec;
+  };
+  const int c = nsm::autotune_tiles(key, time_of, nullptr);
+  return c ? c : nsm::kTileElems;
+}
+
+}  // namespace
+
+extern "C" {
+
 int nsm_model_create(int64_t num_nodes, const double* reference_coordinates, int num_blocks, const int* block_ids,
                      const int64_t* block_num_elem, const int* conn, const double* density,
                      const double* bulk_modulus, const double* shear_modulus, int precision, nsm_model** out)
+{
+  return model_create(num_nodes, reference_coordinates, num_blocks, block_ids, block_num_elem, conn, density,
+                      bulk_modulus, shear_modulus, precision, 0, out);
+}
+
+int nsm_model_tile_elems(nsm_model* md, int* tile_elems)
+{
+  if (!md || !tile_elems) return fail(NSM_ERR_ARG, "null argument");
+  *tile_elems = md->tiled ? md->tiles.tile_elems : 0;
+  return NSM_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+int model_create(int64_t num_nodes, const double* reference_coordinates, int num_blocks, const int* block_ids,
+                 const int64_t* block_num_elem, const int* conn, const double* density, const double* bulk_modulus,
+                 const double* shear_modulus, int precision, int tile_override, nsm_model** out)
 {
   if (!out) return fail(NSM_ERR_ARG, "out is null");
   *out = nullptr;
@@ -876,15 +947,12 @@ int nsm_model_create(int64_t num_nodes, const double* reference_coordinates, int
   if (precision == NSM_MIXED && E > 0) {
     const char* mode = std::getenv("NSM_MIXED_ASSEMBLY");
     if (!(mode && std::strcmp(mode, "slots") == 0)) {
-      // default 512-element tiles (eight waves, one workgroup per CU);
-      // NSM_TILE_ELEMS=256, 128 or 64 selects smaller ones (four, two, one
-      // waves per workgroup)
-      const char* te         = std::getenv("NSM_TILE_ELEMS");
-      const int   tile_elems = !te                          ? nsm::kTileElems
-                               : std::strcmp(te, "256") == 0 ? 256
-                               : std::strcmp(te, "128") == 0 ? 128
-                               : std::strcmp(te, "64") == 0  ? 64
-                                                             : nsm::kTileElems;
+      // 64 to 512 elements per tile (one to eight waves per workgroup):
+      // NSM_TILE_ELEMS, else 512 on large meshes, else the autotuner's choice
+      const int tile_elems = tile_override > 0 ? tile_override
+                                               : model_tile_elems(num_nodes, reference_coordinates, num_blocks,
+                                                                  block_ids, block_num_elem, conn, density,
+                                                                  bulk_modulus, shear_modulus, E);
       std::string err;
       md->tiled = nsm::build_tiles(num_nodes, reference_coordinates, conn, off, tile_elems, md->tiles, &err);
       if (!md->tiled) md->tiles = nsm::TileSet();
@@ -983,6 +1051,10 @@ int nsm_model_create(int64_t num_nodes, const double* reference_coordinates, int
   *out = md;
   return NSM_OK;
 }
+
+}  // namespace
+
+extern "C" {
 
 int nsm_model_destroy(nsm_model* md)
 {

@@ -164,6 +164,15 @@ class Ensemble:
         return dict(num_samples=a.value, num_nodes=b.value, num_elem=c.value, work_items=d.value,
                     tile_elems=t.value, device_bytes=f.value)
 
+    def AutotuneTimings(self) -> Dict[int, float]:
+        """Seconds per step of each candidate tile size the autotuner timed at
+        setup (empty when the choice was cached or given)."""
+        cnt = C.c_int()
+        te = np.zeros(8, np.int32)
+        sec = np.zeros(8)
+        check(lib().nsm_ensemble_autotune_timings(self._h, 8, C.byref(cnt), te.ctypes.data, sec.ctypes.data))
+        return {int(te[k]): float(sec[k]) for k in range(min(cnt.value, 8))}
+
     def EnableKernelTiming(self, enable=True):
         check(lib().nsm_ensemble_enable_kernel_timing(self._h, int(enable)))
 

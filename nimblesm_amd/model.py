@@ -258,6 +258,12 @@ class ModelData:
         return dict(tiled=bool(t.value), num_tiles=a.value, interior_nodes=b.value, boundary_nodes=c.value,
                     gather_tiles=g.value)
 
+    def TileElems(self) -> int:
+        """Elements per element tile of a tiled mixed model (0 otherwise)."""
+        t = C.c_int()
+        check(lib().nsm_model_tile_elems(self._h, C.byref(t)))
+        return t.value
+
     # -- reaction-force history
     def SetReactionNodes(self, nodes, coordinate=2):
         n = np.ascontiguousarray(nodes, dtype=np.int32)

@@ -18,8 +18,10 @@ EXE = os.path.join(ROOT, "tests", "cpp", "adapter_run")
 
 
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
-def test_cpp_adapter_run_matches_python(hip_device, tmp_path, precision):
+def test_cpp_adapter_run_matches_python(hip_device, tmp_path, monkeypatch, precision):
     assert os.path.exists(EXE), "tests/cpp/adapter_run is missing: run __graft_entry__.build()"
+    # one tile size for both processes (each would otherwise time its own)
+    monkeypatch.setenv("NSM_TILE_ELEMS", "128")
     n, steps = 8, 30
     prob = N.CubeProblem(n=n, materials=[N.problems.STEEL, N.problems.ALUMINIUM], num_steps=steps)
     out = tmp_path / "run.bin"

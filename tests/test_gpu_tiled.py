@@ -61,11 +61,12 @@ def run_both(X, conn, blk_off, mats, steps, precision="mixed"):
     return m, run
 
 
-def test_assembly_info_and_element_order(hip_device):
+def test_assembly_info_and_element_order(hip_device, monkeypatch):
+    monkeypatch.setenv("NSM_TILE_ELEMS", "512")
     prob = N.CubeProblem(n=16, materials=[N.problems.STEEL, N.problems.ALUMINIUM], num_steps=1)
     mixed = prob.make_model("mixed")
     info = mixed.AssemblyInfo()
-    assert info["tiled"] and info["num_tiles"] == 8  # 512-element tiles
+    assert info["tiled"] and info["num_tiles"] == 8 and mixed.TileElems() == 512
     assert info["interior_nodes"] + info["boundary_nodes"] == prob.num_nodes
     order = mixed.ElementOrder()
     assert np.array_equal(np.sort(order), np.arange(prob.num_elem))
@@ -159,8 +160,8 @@ def test_tiled_run_with_ten_blocks(hip_device):
 
 @pytest.mark.parametrize("tile_elems", [64, 128, 256])
 def test_other_tile_sizes_against_oracle(hip_device, monkeypatch, tile_elems):
-    """NSM_TILE_ELEMS=64 / 128 / 256 (1, 2 or 4 waves per tile; the default
-    is 512, 8 waves): tile sizes, a run and the tile-blocked F / sigma against
+    """NSM_TILE_ELEMS=64 / 128 / 256 (1, 2 or 4 waves per tile; 512 is 8
+    waves): tile sizes, a run and the tile-blocked F / sigma against
     the oracle."""
     monkeypatch.setenv("NSM_TILE_ELEMS", str(tile_elems))
     n = 10
@@ -288,3 +289,39 @@ def test_nodes_without_elements(hip_device, precision):
     else:
         assert rel_err(u[:-3], run.u[:-3]) <= 1e-6 and rel_err(v[:-3], run.v[:-3]) <= 1e-5
     m.close()
+
+
+def test_autotuned_tile_size(hip_device, monkeypatch, tmp_path):
+    """Without NSM_TILE_ELEMS a small mixed model takes the autotuner's tile
+    size (timed steps of each candidate), records it in the NSM_AUTOTUNE_CACHE
+    file, and runs at it against the oracle; an ensemble reports what the
+    autotuner timed."""
+    monkeypatch.delenv("NSM_TILE_ELEMS", raising=False)
+    prob = N.CubeProblem(n=9, materials=[N.problems.STEEL], num_steps=40)
+    m = prob.make_model("mixed")
+    te = m.TileElems()
+    assert te in (64, 128, 256, 512)
+    m2 = prob.make_model("mixed")  # same shape: the cached choice
+    assert m2.TileElems() == te
+    m2.close()
+    integ = prob.make_integrator(m)
+    integ.Integrate()
+    om = oracle_model_from_problem(prob)
+    run = O.ExplicitRun(om, prob.num_steps * prob.dt_factor * om.critical_dt(), prob.num_steps, 0,
+                        reaction_nodes=prob.top, reaction_coord=2)
+    run.advance(prob.num_steps)
+    assert rel_err(m.GetVectorNodeData("displacement"), run.u) <= 1e-6
+    assert rel_err(m.GetReactionHistory(0, prob.num_steps), run.reaction) <= 1e-5
+    m.close()
+    from nimblesm_amd.ensemble import EnsembleProblem, draw_parameters
+
+    cache = tmp_path / "tiles.txt"
+    monkeypatch.setenv("NSM_AUTOTUNE_CACHE", str(cache))
+    ep = EnsembleProblem(draw_parameters(40, seed=5, n=(10, 12)), num_steps=20).build()
+    t = ep.ens.AutotuneTimings()
+    print(f"autotuner: model 9^3 -> {te}; ensemble {ep.ens.Info()['tile_elems']} from "
+          + ", ".join(f"{k}: {1e6 * v:.1f} us" for k, v in t.items()), flush=True)
+    assert sorted(t) == [64, 128, 256, 512] and all(v > 0 for v in t.values())
+    assert ep.ens.Info()["tile_elems"] == min(t, key=t.get)
+    assert cache.read_text().split()[-1] == str(ep.ens.Info()["tile_elems"])
+    ep.close()
