@@ -413,6 +413,9 @@ def main():
     ap.add_argument("--fp64-steps", type=int, default=20)
     ap.add_argument("--no-ensemble", action="store_true", help="skip the batched-ensemble leg")
     ap.add_argument("--no-surrogate", action="store_true", help="skip the learned-control-variate leg")
+    ap.add_argument("--only", choices=["ensemble", "surrogate"], default=None,
+                    help="run only that leg and print its object (for profiling)")
+    ap.add_argument("--surrogate-iterations", type=int, default=3000)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run 5 device copies of MIB MiB (calibrates rocprofv3 FETCH_SIZE / WRITE_SIZE)")
     args = ap.parse_args()
@@ -439,6 +442,13 @@ def main():
             dst.copy_(src)
         torch.cuda.synchronize()
         print(json.dumps({"copy_calibration_bytes": nbytes, "copies": 5}), flush=True)
+        return
+
+    if args.only == "ensemble":
+        print(json.dumps({"ensemble": ensemble_leg(N)}), flush=True)
+        return
+    if args.only == "surrogate":
+        print(json.dumps({"surrogate": surrogate_leg(N, iterations=args.surrogate_iterations)}), flush=True)
         return
 
     prob = N.workload(args.workload).build()
@@ -560,7 +570,7 @@ def main():
     if not args.no_ensemble and not args.uq:
         out["ensemble"] = ensemble_leg(N)
     if not args.no_surrogate and not args.uq:
-        out["surrogate"] = surrogate_leg(N)
+        out["surrogate"] = surrogate_leg(N, iterations=args.surrogate_iterations)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(prob, args.cpu_steps)
     print(json.dumps(out), flush=True)

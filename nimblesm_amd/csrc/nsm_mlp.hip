@@ -69,6 +69,9 @@ __device__ __forceinline__ float gelu_grad(float x)
 // 4 (l / 16) .. 4 (l / 16) + 3.
 // ---------------------------------------------------------------------------
 constexpr int kTile = 64, kKStep = 32, kPad = 8, kGemmThreads = 256;
+// weight-gradient products (K = batch) are split into at most this many
+// K ranges so that more workgroups share the long reduction
+constexpr int kMaxSplits = 8;
 
 enum Epilogue {
   kStoreC     = 0,  // C = v (FP32)
@@ -77,45 +80,104 @@ enum Epilogue {
   kBackHidden = 3,  // g = v * gelu'(Z): G = g (FP32), D = bf16(g)
 };
 
+// How a 64 x 32 operand tile is staged: 16-byte loads along k (contiguous
+// in k), 16-byte loads along m / n written transposed into LDS (contiguous
+// in m / n), or one element at a time.
+enum Staging { kAlongK = 1, kAlongMN = 2, kScalar = 0 };
+
 struct GemmArgs {
   int          M, N, K;
   const bf16*  a;
   int64_t      a_m, a_k;
   const bf16*  b;
   int64_t      b_k, b_n;
-  float*       c;  // [M][ldc]
+  float*       c;  // [M][ldc]; split-K: split z writes c + z * split_stride
   int64_t      ldc;
   const float* bias;  // [N]
   bf16*        h;     // [M][ldc]
   const float* z;     // [M][ldc]: pre-activation (kBackHidden)
+  int          a_stage, b_stage;
+  int          k_chunk;  // K range of one split (multiple of kKStep)
+  int64_t      split_stride;
 };
+
+// Staging of one operand tile (64 rows r of the M or N side, 32 k) into
+// registers: thread t holds 8 values.  AlongK: row t / 4, k = 8 (t % 4) + j;
+// AlongMN: k = t / 8, rows 8 (t % 8) + j.
+struct StageRegs {
+  bf16x8 v;
+};
+
+__device__ __forceinline__ StageRegs stage_load(const bf16* p, int64_t s_r, int64_t s_k, int mode, int r0, int R,
+                                                int k0, int K)
+{
+  const int t = (int)threadIdx.x;
+  StageRegs g;
+  if (mode == kAlongK) {
+    const int r = r0 + (t >> 2), k = k0 + (t & 3) * 8;
+    if (r < R && k + 8 <= K) {
+      g.v = *reinterpret_cast<const bf16x8*>(p + (int64_t)r * s_r + k);
+      return g;
+    }
+#pragma unroll
+    for (int j = 0; j < 8; j++) g.v[j] = (r < R && k + j < K) ? p[(int64_t)r * s_r + (k + j)] : (bf16)0.0f;
+  } else if (mode == kAlongMN) {
+    const int k = k0 + (t >> 3), r = r0 + (t & 7) * 8;
+    if (k < K && r + 8 <= R) {
+      g.v = *reinterpret_cast<const bf16x8*>(p + (int64_t)k * s_k + r);
+      return g;
+    }
+#pragma unroll
+    for (int j = 0; j < 8; j++) g.v[j] = (k < K && r + j < R) ? p[(int64_t)k * s_k + (r + j)] : (bf16)0.0f;
+  } else {
+    const int r = r0 + (t >> 2), k = k0 + (t & 3) * 8;
+#pragma unroll
+    for (int j = 0; j < 8; j++)
+      g.v[j] = (r < R && k + j < K) ? p[(int64_t)r * s_r + (int64_t)(k + j) * s_k] : (bf16)0.0f;
+  }
+  return g;
+}
+
+template <int kLd>
+__device__ __forceinline__ void stage_store(const StageRegs& g, int mode, bf16 (*S)[kLd])
+{
+  const int t = (int)threadIdx.x;
+  if (mode == kAlongMN) {
+    const int k = t >> 3, r = (t & 7) * 8;
+#pragma unroll
+    for (int j = 0; j < 8; j++) S[r + j][k] = g.v[j];
+  } else {
+    *reinterpret_cast<bf16x8*>(&S[t >> 2][(t & 3) * 8]) = g.v;
+  }
+}
 
 __global__ __launch_bounds__(kGemmThreads) void k_gemm_bf16(GemmArgs g, int mode)
 {
-  __shared__ __attribute__((aligned(16))) bf16 As[kTile][kKStep + kPad];
-  __shared__ __attribute__((aligned(16))) bf16 Bs[kTile][kKStep + kPad];
-  const int tid = (int)threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  constexpr int kLd = kKStep + kPad;
+  __shared__ __attribute__((aligned(16))) bf16 As[kTile][kLd];
+  __shared__ __attribute__((aligned(16))) bf16 Bs[kTile][kLd];
+  const int lane = (int)threadIdx.x & 63, wave = (int)threadIdx.x >> 6;
   const int m0 = (int)blockIdx.y * kTile, n0 = (int)blockIdx.x * kTile;
   const int wm = wave >> 1, wn = wave & 1;
-  // staging: thread t moves 8 consecutive k of row (A) / column (B) t / 4
-  const int sr = tid >> 2, sk = (tid & 3) * 8;
+  // split z of the K range
+  const int kb = (int)blockIdx.z * g.k_chunk, ke = min(g.K, kb + g.k_chunk);
   f32x4     acc[2][2];
 #pragma unroll
   for (int i = 0; i < 2; i++)
 #pragma unroll
     for (int j = 0; j < 2; j++) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
 
-  for (int k0 = 0; k0 < g.K; k0 += kKStep) {
-    {
-      const int m = m0 + sr, n = n0 + sr;
-#pragma unroll
-      for (int j = 0; j < 8; j++) {
-        const int k = k0 + sk + j;
-        As[sr][sk + j] = (m < g.M && k < g.K) ? g.a[(int64_t)m * g.a_m + (int64_t)k * g.a_k] : (bf16)0.0f;
-        Bs[sr][sk + j] = (n < g.N && k < g.K) ? g.b[(int64_t)k * g.b_k + (int64_t)n * g.b_n] : (bf16)0.0f;
-      }
-    }
+  // the next K step's operands are loaded while the current one multiplies
+  StageRegs ra = stage_load(g.a, g.a_m, g.a_k, g.a_stage, m0, g.M, kb, ke);
+  StageRegs rb = stage_load(g.b, g.b_n, g.b_k, g.b_stage, n0, g.N, kb, ke);
+  for (int k0 = kb; k0 < ke; k0 += kKStep) {
+    stage_store<kLd>(ra, g.a_stage, As);
+    stage_store<kLd>(rb, g.b_stage, Bs);
     __syncthreads();
+    if (k0 + kKStep < ke) {
+      ra = stage_load(g.a, g.a_m, g.a_k, g.a_stage, m0, g.M, k0 + kKStep, ke);
+      rb = stage_load(g.b, g.b_n, g.b_k, g.b_stage, n0, g.N, k0 + kKStep, ke);
+    }
     bf16x8 af[2], bfr[2];
 #pragma unroll
     for (int i = 0; i < 2; i++) {
@@ -129,6 +191,7 @@ __global__ __launch_bounds__(kGemmThreads) void k_gemm_bf16(GemmArgs g, int mode
     __syncthreads();
   }
 
+  float* const c = g.c + (int64_t)blockIdx.z * g.split_stride;
 #pragma unroll
   for (int i = 0; i < 2; i++)
 #pragma unroll
@@ -141,19 +204,30 @@ __global__ __launch_bounds__(kGemmThreads) void k_gemm_bf16(GemmArgs g, int mode
         const int64_t o = (int64_t)m * g.ldc + n;
         float         v = acc[i][j][r];
         if (mode == kStoreC) {
-          g.c[o] = v;
+          c[o] = v;
         } else if (mode == kHidden) {
           v += g.bias[n];
-          g.c[o] = v;
+          c[o]   = v;
           g.h[o] = (bf16)gelu(v);
         } else if (mode == kOutput) {
-          g.c[o] = v + g.bias[n];
+          c[o] = v + g.bias[n];
         } else {  // kBackHidden
           const float gr = v * gelu_grad(g.z[o]);
-          g.c[o]         = gr;
+          c[o]           = gr;
           g.h[o]         = (bf16)gr;
         }
       }
+}
+
+// out[i] = sum over splits s of part[s * stride + i], ascending s
+__global__ __launch_bounds__(256) void k_split_sum(const float* __restrict__ part, int splits, int64_t stride,
+                                                   int64_t n, float* __restrict__ out)
+{
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  float s = 0.0f;
+  for (int z = 0; z < splits; z++) s += part[(int64_t)z * stride + i];
+  out[i] = s;
 }
 
 // FP32 -> BF16 (round to nearest even), n values
@@ -163,15 +237,17 @@ __global__ __launch_bounds__(256) void k_to_bf16(const float* __restrict__ x, bf
   if (i < n) y[i] = (bf16)x[i];
 }
 
-// Mean squared error over B x d outputs (fixed-order sum in one workgroup)
-// and its gradient G = 2 (y - t) / (B d), also as BF16.
+// Gradient G = 2 (y - t) / n of the mean squared error over n outputs (also
+// as BF16), and per workgroup the sum of squared errors (fixed tree).
+constexpr int kMseBlocks = 64;
 __global__ __launch_bounds__(256) void k_mse(const float* __restrict__ y, const float* __restrict__ t, int64_t n,
-                                             float* __restrict__ grad, bf16* __restrict__ grad_bf, double* loss_out)
+                                             float* __restrict__ grad, bf16* __restrict__ grad_bf,
+                                             double* __restrict__ partial)
 {
   __shared__ double red[256];
   double            s     = 0.0;
   const float       scale = 2.0f / (float)n;
-  for (int64_t i = threadIdx.x; i < n; i += 256) {
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
     const float e = y[i] - t[i];
     s += (double)e * (double)e;
     const float gr = scale * e;
@@ -184,18 +260,32 @@ __global__ __launch_bounds__(256) void k_mse(const float* __restrict__ y, const 
     if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
     __syncthreads();
   }
-  if (threadIdx.x == 0 && loss_out) *loss_out = red[0] / (double)n;
+  if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
 }
 
-// db[n] = sum over the batch of G[b][n], ascending b
+// loss = sum of the kMseBlocks partials (ascending) / n
+__global__ __launch_bounds__(64) void k_mse_total(const double* __restrict__ partial, int64_t n,
+                                                  double* __restrict__ loss_out)
+{
+  if (threadIdx.x != 0) return;
+  double s = 0.0;
+  for (int b = 0; b < kMseBlocks; b++) s += partial[b];
+  *loss_out = s / (double)n;
+}
+
+// db[n] = sum over the batch of G[b][n]: one wave per column, lane l sums
+// rows l, l + 64, ... in ascending order, then a fixed shuffle tree
 __global__ __launch_bounds__(256) void k_bias_grad(const float* __restrict__ G, int64_t B, int N,
                                                    float* __restrict__ db)
 {
-  const int n = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  const int n = (int)(blockIdx.x * 4 + (threadIdx.x >> 6));
   if (n >= N) return;
-  float s = 0.0f;
-  for (int64_t b = 0; b < B; b++) s += G[b * N + n];
-  db[n] = s;
+  const int lane = (int)threadIdx.x & 63;
+  float     s    = 0.0f;
+  for (int64_t b = lane; b < B; b += 64) s += G[b * N + n];
+#pragma unroll
+  for (int w = 32; w > 0; w >>= 1) s += __shfl_down(s, w, 64);
+  if (lane == 0) db[n] = s;
 }
 
 // Adam on the FP32 master parameters; BF16 copies for the forward pass.
@@ -246,6 +336,9 @@ struct nsm_mlp {
   float*               t32 = nullptr;  // targets [cap][d_L]
   double*              loss = nullptr; // [loss_cap]
   int64_t              loss_cap = 0;
+  double*              loss_part = nullptr;  // [kMseBlocks]
+  float*               split = nullptr;      // split-K partial products
+  int64_t              split_cap = 0;
 
   int L() const { return (int)width.size() - 1; }
   int max_width() const { return *std::max_element(width.begin(), width.end()); }
@@ -253,11 +346,39 @@ struct nsm_mlp {
 
 namespace {
 
-int mlp_gemm(nsm_mlp* net, const GemmArgs& a, int mode)
+// staging mode of an operand with rows at stride s_r and k at stride s_k
+int staging(const bf16* p, int64_t s_r, int64_t s_k)
 {
-  const dim3 grid(blocks_of(a.N, kTile), blocks_of(a.M, kTile));
+  const bool aligned = ((uintptr_t)p % 16) == 0;
+  if (aligned && s_k == 1 && s_r % 8 == 0) return kAlongK;
+  if (aligned && s_r == 1 && s_k % 8 == 0) return kAlongMN;
+  return kScalar;
+}
+
+// C = A B with the epilogue `mode`.  splits > 1 (kStoreC only): the K range
+// is cut into `splits` parts written to net->split (one M x N block each)
+// and summed in ascending order into a.c.
+int mlp_gemm(nsm_mlp* net, GemmArgs a, int mode, int splits = 1)
+{
+  a.a_stage = staging(a.a, a.a_m, a.a_k);
+  a.b_stage = staging(a.b, a.b_n, a.b_k);
+  splits    = std::max(1, std::min(splits, (a.K + kKStep - 1) / kKStep));
+  a.k_chunk = ((a.K + splits - 1) / splits + kKStep - 1) / kKStep * kKStep;
+  float* final_c = a.c;
+  const int64_t mn = (int64_t)a.M * a.ldc;
+  if (splits > 1) {
+    if (mode != kStoreC || mn * splits > net->split_cap) return fail(NSM_ERR_STATE, "split-K scratch too small");
+    a.c            = net->split;
+    a.split_stride = mn;
+  }
+  const dim3 grid(blocks_of(a.N, kTile), blocks_of(a.M, kTile), (unsigned)splits);
   hipLaunchKernelGGL(k_gemm_bf16, grid, dim3(kGemmThreads), 0, net->stream, a, mode);
   MLP_LAUNCHED();
+  if (splits > 1) {
+    hipLaunchKernelGGL(k_split_sum, dim3(blocks_of(mn, 256)), dim3(256), 0, net->stream, net->split, splits, mn, mn,
+                       final_c);
+    MLP_LAUNCHED();
+  }
   return NSM_OK;
 }
 
@@ -301,14 +422,17 @@ int ensure_batch(nsm_mlp* net, int64_t B)
   return NSM_OK;
 }
 
-// Forward pass of B rows already in x32; output Z_L in z[L-1].
-int forward(nsm_mlp* net, int64_t B)
+// Forward pass of B rows already in x32 (converted to BF16 into h[0] unless
+// convert_input is false: training converts once); output Z_L in z[L-1].
+int forward(nsm_mlp* net, int64_t B, bool convert_input = true)
 {
   const int L  = net->L();
   const int d0 = net->width[0];
-  hipLaunchKernelGGL(k_to_bf16, dim3(blocks_of(B * d0, 256)), dim3(256), 0, net->stream, net->x32, net->h[0],
-                     B * d0);
-  MLP_LAUNCHED();
+  if (convert_input) {
+    hipLaunchKernelGGL(k_to_bf16, dim3(blocks_of(B * d0, 256)), dim3(256), 0, net->stream, net->x32, net->h[0],
+                       B * d0);
+    MLP_LAUNCHED();
+  }
   for (int l = 1; l <= L; l++) {
     const int din = net->width[(size_t)l - 1], dout = net->width[(size_t)l];
     GemmArgs  a{};
@@ -337,9 +461,13 @@ int backward(nsm_mlp* net, int64_t B, double* loss_out)
 {
   const int L  = net->L();
   const int dL = net->width[(size_t)L];
-  hipLaunchKernelGGL(k_mse, dim3(1), dim3(256), 0, net->stream, net->z[(size_t)L - 1], net->t32, B * dL, net->gf,
-                     net->gb, loss_out);
+  hipLaunchKernelGGL(k_mse, dim3(kMseBlocks), dim3(256), 0, net->stream, net->z[(size_t)L - 1], net->t32, B * dL,
+                     net->gf, net->gb, net->loss_part);
   MLP_LAUNCHED();
+  if (loss_out) {
+    hipLaunchKernelGGL(k_mse_total, dim3(1), dim3(64), 0, net->stream, net->loss_part, B * dL, loss_out);
+    MLP_LAUNCHED();
+  }
   float* gcur = net->gf;
   bf16*  dcur = net->gb;
   float* gnext = net->gf2;
@@ -359,9 +487,10 @@ int backward(nsm_mlp* net, int64_t B, double* loss_out)
     a.b_n = 1;
     a.c   = net->g + net->w_off[(size_t)l - 1];
     a.ldc = din;
-    int st = mlp_gemm(net, a, kStoreC);
+    // about 64 batch rows per split
+    int st = mlp_gemm(net, a, kStoreC, (int)std::min<int64_t>(kMaxSplits, std::max<int64_t>(1, B / 64)));
     if (st) return st;
-    hipLaunchKernelGGL(k_bias_grad, dim3(blocks_of(dout, 256)), dim3(256), 0, net->stream, gcur, B, dout,
+    hipLaunchKernelGGL(k_bias_grad, dim3(blocks_of(dout, 4)), dim3(256), 0, net->stream, gcur, B, dout,
                        net->g + net->b_off[(size_t)l - 1]);
     MLP_LAUNCHED();
     if (l == 1) break;
@@ -450,9 +579,14 @@ int nsm_mlp_create(int num_layers, const int* widths, uint64_t seed, nsm_mlp** o
   if (hipStreamCreateWithFlags(&net->stream, hipStreamNonBlocking) != hipSuccess)
     return cleanup(fail(NSM_ERR_HIP, "hipStreamCreate failed"));
   const size_t n = (size_t)net->num_params;
+  int64_t max_w = 0;
+  for (int l = 1; l <= num_layers; l++) max_w = std::max<int64_t>(max_w, (int64_t)widths[l] * widths[l - 1]);
+  net->split_cap = kMaxSplits * max_w;
   if (hipMalloc((void**)&net->p, n * 4) != hipSuccess || hipMalloc((void**)&net->pb, n * 2) != hipSuccess ||
       hipMalloc((void**)&net->g, n * 4) != hipSuccess || hipMalloc((void**)&net->m1, n * 4) != hipSuccess ||
-      hipMalloc((void**)&net->m2, n * 4) != hipSuccess)
+      hipMalloc((void**)&net->m2, n * 4) != hipSuccess ||
+      hipMalloc((void**)&net->split, (size_t)net->split_cap * 4) != hipSuccess ||
+      hipMalloc((void**)&net->loss_part, kMseBlocks * sizeof(double)) != hipSuccess)
     return cleanup(fail(NSM_ERR_NOMEM, "device allocation of the parameters failed"));
   // uniform(-a, a) weights with a = sqrt(6 / (fan_in + fan_out)), zero biases
   std::vector<float> init(n, 0.0f);
@@ -476,7 +610,8 @@ int nsm_mlp_destroy(nsm_mlp* net)
   (void)hipSetDevice(net->device);
   if (net->stream) (void)hipStreamSynchronize(net->stream);
   free_batch(net);
-  for (void* q : {(void*)net->p, (void*)net->pb, (void*)net->g, (void*)net->m1, (void*)net->m2, (void*)net->loss})
+  for (void* q : {(void*)net->p, (void*)net->pb, (void*)net->g, (void*)net->m1, (void*)net->m2, (void*)net->loss,
+                  (void*)net->loss_part, (void*)net->split})
     if (q) (void)hipFree(q);
   if (net->stream) (void)hipStreamDestroy(net->stream);
   delete net;
@@ -562,7 +697,7 @@ int nsm_mlp_train(nsm_mlp* net, int64_t batch, const float* x, const float* t, i
     net->adam_t++;
     const float c1 = (float)(1.0 - std::pow((double)b1, (double)net->adam_t));
     const float c2 = (float)(1.0 - std::pow((double)b2, (double)net->adam_t));
-    if ((st = forward(net, batch)) || (st = backward(net, batch, net->loss + it))) return st;
+    if ((st = forward(net, batch, it == 0)) || (st = backward(net, batch, net->loss + it))) return st;
     hipLaunchKernelGGL(k_adam, dim3(blocks_of(net->num_params, 256)), dim3(256), 0, net->stream, net->p, net->g,
                        net->m1, net->m2, net->pb, net->num_params, lri, b1, b2, c1, c2, eps);
     MLP_LAUNCHED();
