@@ -186,7 +186,9 @@ struct nsm_model {
   bool                  have_mass = false;
   double                critical_dt = 0.0;
 
-  bool                                timing = false;
+  bool                                timing       = false;
+  int                                 timing_every = 1;  // time every n-th launch of a kernel
+  int64_t                             timing_count[4] = {0, 0, 0, 0};
   std::vector<EventPair>              event_pool;
   size_t                              event_next = 0;
   std::vector<std::pair<int, size_t>> timed;  // (kernel, event index)
@@ -273,6 +275,8 @@ int time_begin(nsm_model* md, int kernel, size_t* idx)
 {
   *idx = (size_t)-1;
   if (!md->timing) return NSM_OK;
+  // every timing_every-th launch of each kernel is bracketed by events
+  if (kernel >= 0 && kernel < 4 && (md->timing_count[kernel]++ % md->timing_every) != 0) return NSM_OK;
   if (md->event_next == md->event_pool.size()) {
     EventPair p;
     NSM_HIP(hipEventCreate(&p.start));
@@ -1622,7 +1626,10 @@ int nsm_model_enable_kernel_timing(nsm_model* md, int enable)
 {
   if (!md) return fail(NSM_ERR_ARG, "null model");
   NSM_HIP(hipStreamSynchronize(md->stream));
-  md->timing     = enable != 0;
+  if (enable < 0) return fail(NSM_ERR_ARG, "negative timing interval");
+  md->timing       = enable != 0;
+  md->timing_every = enable > 0 ? enable : 1;
+  for (auto& c : md->timing_count) c = 0;
   md->event_next = 0;
   md->timed.clear();
   return NSM_OK;

@@ -312,8 +312,9 @@ class ModelData:
     def Synchronize(self):
         check(lib().nsm_model_synchronize(self._h))
 
-    def EnableKernelTiming(self, enable=True):
-        check(lib().nsm_model_enable_kernel_timing(self._h, int(enable)))
+    def EnableKernelTiming(self, enable=True, every: int = 1):
+        """HIP events around every `every`-th launch of each kernel."""
+        check(lib().nsm_model_enable_kernel_timing(self._h, int(every) if enable else 0))
 
     def KernelTime(self, kernel: str):
         ms, n = C.c_double(), C.c_int64()
