@@ -183,3 +183,58 @@ def test_full_ensemble_workload(hip_device):
             assert m.GetCriticalTimeStep() == res[precision][1][i]
             assert np.array_equal(m.SampleReactionHistory(400, 64), res[precision][0][i]), (n, precision)
             m.close()
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_displacement_and_initial_velocity_conditions(hip_device, precision):
+    """Per-sample magnitudes of an initial velocity, a prescribed
+    displacement, and a prescribed velocity followed by a prescribed
+    displacement on the same nodes, with output steps every 7 steps: each
+    sample bit-identical to a ModelData run with those conditions."""
+    n, steps, of = 6, 30, 7
+    X, conn = N.jittered_cube(n, 99, 0.1)
+    n1 = n + 1
+    ids = np.arange(n1 ** 3, dtype=np.int32)
+    k = ids // (n1 * n1)
+    j = (ids // n1) % n1
+    bottom, top, side = ids[k == 0], ids[k == n], ids[(j == 0) & (k > 0) & (k < n)]
+    conds = [("initial_velocity", ids, 0), ("prescribed_velocity", bottom, 0), ("prescribed_velocity", bottom, 1),
+             ("prescribed_velocity", bottom, 2), ("prescribed_displacement", top, 2),
+             ("prescribed_velocity", side, 1), ("prescribed_displacement", side, 1)]
+    rng = np.random.default_rng(12)
+    S = 3
+    mats = [N.NeohookeanMaterial.from_young(rng.uniform(5000, 9000), rng.uniform(100e9, 250e9),
+                                            rng.uniform(0.25, 0.4)) for _ in range(S)]
+    mags = np.zeros((S, len(conds)))
+    mags[:, 0] = rng.uniform(-0.5, 0.5, S)
+    mags[:, 4] = rng.uniform(1e-5, 3e-5, S)
+    mags[:, 5] = rng.uniform(0.1, 0.3, S)
+    mags[:, 6] = rng.uniform(-2e-5, 2e-5, S)
+    ens = N.Ensemble(precision, 64)
+    mid = ens.AddMesh(X, [conn])
+    for kind, nodes, c in conds:
+        ens.AddBoundaryCondition(mid, kind, nodes, c)
+    ens.SetReactionNodes(mid, top, 2)
+    ens.AddSamples(mid, [m.density for m in mats], [m.bulk_modulus for m in mats], [m.shear_modulus for m in mats],
+                   mags)
+    ens.Setup()
+    dts = ens.CriticalTimeSteps()
+    ens.Run(steps, steps * (0.5 * dts), output_frequency=of, use_graph=True)
+    os.environ["NSM_TILE_ELEMS"] = "64"
+    try:
+        for s in range(S):
+            m = N.ModelData(X, [N.Block(1, conn, mats[s])], precision)
+            for (kind, nodes, c), v in zip(conds, mags[s]):
+                m.AddBoundaryCondition(kind, nodes, c, float(v))
+            m.SetReactionNodes(top, 2)
+            m.ComputeLumpedMass()
+            assert m.GetCriticalTimeStep() == dts[s]
+            integ = N.ExplicitTimeIntegrator(m, steps * (0.5 * dts[s]), steps, of).Initialize()
+            integ.Integrate()
+            for f in FIELDS:
+                assert np.array_equal(ens.GetNodeData(s, f), m.GetVectorNodeData(f)), (s, f)
+            assert np.array_equal(ens.GetReactionHistory(s), m.GetReactionHistory(0, steps)), s
+            m.close()
+    finally:
+        del os.environ["NSM_TILE_ELEMS"]
+    ens.close()
