@@ -396,6 +396,7 @@ def surrogate_leg(N, num_steps=400, n_train=512, n_eval=8192, fraction=0.05, ite
         "iterations": iterations, "final_loss": float(mlp.loss[-1]), "initial_loss": float(mlp.loss[0]),
         "samples_per_s": sps, "wall_s": total, "phases": phases, "autotune_s": autotune_s,
         "qoi_mean_rel_err": err, "judged_rel_err": est.est_rel_err, "heldout_rel_err": est.heldout_rel_err,
+        "heldout_mean_rel_err": est.heldout_mean_rel_err,
         "fallback": est.fallback, "plain_solves": n_plain,
         "equal_cost_plain_mc_rel_err": err_mc, "equal_cost_linear_cv_rel_err": err_lin,
         "plain_one_at_a_time": plain,

@@ -773,7 +773,10 @@ int nsm_ensemble_add_mesh(nsm_ensemble* es, int64_t num_nodes, const double* X, 
     m.blk_off.push_back(m.blk_off.back() + block_num_elem[b]);
   }
   m.E = m.blk_off.back();
-  if (m.E > 0 && !conn) return fail(NSM_ERR_ARG, "conn is null");
+  // every node is updated by the element or node launches of its sample,
+  // which a mesh without elements has none of
+  if (m.E <= 0) return fail(NSM_ERR_ARG, "an ensemble mesh needs at least one element");
+  if (!conn) return fail(NSM_ERR_ARG, "conn is null");
   for (int64_t k = 0; k < 8 * m.E; k++)
     if (conn[k] < 0 || conn[k] >= num_nodes)
       return fail(NSM_ERR_ARG, "connectivity entry %lld = %d out of range", (long long)k, conn[k]);
