@@ -115,13 +115,23 @@ int nsm_neohookean_get_stress(int64_t num_points, double bulk_modulus, double sh
  * value (e, q, c) at F_np1[e*F_elem_stride + q*F_ipt_stride + c*F_comp_stride],
  * sigma value at the same form with the sigma strides (device pointers).
  * Kokkos LayoutLeft [E][8][9] views have strides (1, E, 8E), LayoutRight
- * ones (72, 9, 1); a model's FP64 arrays (nsm_model_field_device_pointer)
- * are [comp][ipt][E]: block elements [e0, e0+n) at base + e0 with strides
- * (1, E, 8E). */
+ * ones (72, 9, 1).  A model's FP64 arrays store elements in groups of 8: use
+ * nsm_neohookean_get_stress_grouped on them. */
 int nsm_neohookean_get_stress_strided(int64_t num_elem, int num_ipt, double bulk_modulus, double shear_modulus,
                                       const double* F_np1, int64_t F_elem_stride, int64_t F_ipt_stride,
                                       int64_t F_comp_stride, double* sigma_np1, int64_t sigma_elem_stride,
                                       int64_t sigma_ipt_stride, int64_t sigma_comp_stride, void* stream);
+/* The same GetStress over views whose elements are stored in groups of 8
+ * (the same reference interface as nsm_neohookean_get_stress_strided):
+ * elements e in [first_elem, first_elem + num_elem), F value (e, q, c) at
+ * F_np1[(e/8)*g + (e%8)*s_e + q*s_q + c*s_c] with F_strides = {g, s_e, s_q,
+ * s_c}, sigma likewise.  A model's FP64 arrays (nsm_model_field_device_pointer,
+ * base pointers) have F strides {576, 1, 8, 64} and sigma strides
+ * {384, 1, 8, 64}; a block's elements [e0, e0 + n) are first_elem = e0,
+ * num_elem = n.  Plain strides are the case g = 8 s_e, first_elem = 0. */
+int nsm_neohookean_get_stress_grouped(int64_t first_elem, int64_t num_elem, int num_ipt, double bulk_modulus,
+                                      double shear_modulus, const double* F_np1, const int64_t* F_strides,
+                                      double* sigma_np1, const int64_t* sigma_strides, void* stream);
 
 /* ---------------------------------------------------------------------------
  * Model-data seam: replaces nimble::ModelDataBase / nimble_kokkos::ModelData
@@ -222,8 +232,10 @@ int nsm_model_set_field(nsm_model* model, int field, const double* in);
  * mode and float in mixed mode) and its element size in bytes.  Node fields
  * are [N][3] / [N] in the node order of nsm_model_get_node_order.  Ipt
  * fields (nc = 9 or 6 components, device elements in the order of
- * nsm_model_get_element_order) are element-fastest [comp][ipt][E] in FP64 and
- * untiled mixed models; tiled mixed models store them per element tile: the
+ * nsm_model_get_element_order) are stored in groups of 8 elements in FP64
+ * models: value (c, q) of element e at 64 nc (e/8) + 64 c + 8 q + e%8 (E
+ * rounded up to 8 elements allocated); untiled mixed models store them
+ * element-fastest [comp][ipt][E]; tiled mixed models per element tile: the
  * tile holding device elements [e0, e0 + n) owns [8 nc e0, 8 nc (e0 + n)),
  * value (c, q) of its element e0 + l at 8 nc e0 + (4 c + q/2) 2n + 2l + q%2
  * (rows of 2n values: the element's points q and q+1, q even, side by side). */

@@ -82,6 +82,16 @@ class NeohookeanMaterial {
                                             F_strides[1], F_strides[2], sigma_np1, sigma_strides[0],
                                             sigma_strides[1], sigma_strides[2], stream));
   }
+  // Arrays with elements in groups of 8 at (group, element, point, component)
+  // strides, elements [first_elem, first_elem + num_elem): a block of an FP64
+  // model's STEP_NP1 arrays.
+  void GetStressGrouped(int64_t first_elem, int64_t num_elem, int num_ipt, const double* F_np1,
+                        const int64_t F_strides[4], double* sigma_np1, const int64_t sigma_strides[4],
+                        void* stream = nullptr) const
+  {
+    check(nsm_neohookean_get_stress_grouped(first_elem, num_elem, num_ipt, bulk_modulus_, shear_modulus_, F_np1,
+                                            F_strides, sigma_np1, sigma_strides, stream));
+  }
 
  private:
   double density_, bulk_modulus_, shear_modulus_;

@@ -16,7 +16,7 @@
 //       the stress plug-in seam (src/nimble_block_material_interface_base.h:76-84,
 //       src/nimble_block_material_interface_factory_base.h:59-72): per block,
 //       the Neo-Hookean stress of the model's STEP_NP1 deformation gradients
-//       on the device (nsm_neohookean_get_stress_strided).  Returned by
+//       on the device (nsm_neohookean_get_stress_grouped).  Returned by
 //       NimbleApplication::CreateBlockMaterialInterfaceFactory
 //       (src/nimble.h:85, src/nimble.cc:280-283) in a NimbleSM application
 //       that installs it.
@@ -158,12 +158,13 @@ class AmdModelData : public nimble::ModelDataBase {
   void UpdateStates(const nimble::DataManager&) override { device().UpdateStates(); }
 
   // -- for the stress plug-in: the device STEP_NP1 arrays of block `id`
-  //    ([comp][ipt][E] over all blocks, FP64) and the stream to run on
+  //    (FP64, elements of all blocks in groups of 8) and the stream to run on
   struct BlockIpt {
     const double* F;
     double*       sigma;
-    int64_t       strides[3];  // element, point, component
-    int64_t       num_elem;
+    int64_t       F_strides[4];      // group of 8 elements, element, point, component
+    int64_t       sigma_strides[4];
+    int64_t       first_elem, num_elem;
   };
   BlockIpt BlockIntegrationPointData(int id)
   {
@@ -174,7 +175,7 @@ class AmdModelData : public nimble::ModelDataBase {
     int64_t       ne = 0;
     for (const auto& b : block_data_)
       if (b.id == id) ne = b.num_elems;
-    return {F + e0, S + e0, {1, num_elem_, 8 * num_elem_}, ne};
+    return {F, S, {576, 1, 8, 64}, {384, 1, 8, 64}, e0, ne};
   }
   void* Stream() { return device().Stream(); }
   ModelData& device()
@@ -209,8 +210,8 @@ class AmdBlockMaterialInterface : public nimble::BlockMaterialInterfaceBase {
       const AmdModelData::BlockIpt ip = model_->BlockIntegrationPointData(b.id);
       const NeohookeanMaterial     mat(b.material_device->GetDensity(), b.material_device->GetBulkModulus(),
                                        b.material_device->GetShearModulus());
-      mat.GetStressStrided(ip.num_elem, b.num_points_per_elem, ip.F, ip.strides, ip.sigma, ip.strides,
-                           model_->Stream());
+      mat.GetStressGrouped(ip.first_elem, ip.num_elem, b.num_points_per_elem, ip.F, ip.F_strides, ip.sigma,
+                           ip.sigma_strides, model_->Stream());
     }
   }
 

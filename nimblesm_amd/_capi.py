@@ -39,6 +39,7 @@ SIGNATURES = {
     "nsm_hex_compute_characteristic_length": (_I, [_I64, _P, _P, _P]),
     "nsm_neohookean_get_stress": (_I, [_I64, _D, _D, _P, _P, _P]),
     "nsm_neohookean_get_stress_strided": (_I, [_I64, _I, _D, _D, _P, _I64, _I64, _I64, _P, _I64, _I64, _I64, _P]),
+    "nsm_neohookean_get_stress_grouped": (_I, [_I64, _I64, _I, _D, _D, _P, _P, _P, _P, _P]),
     "nsm_model_create": (_I, [_I64, _P, _I, _P, _P, _P, _P, _P, _P, _I, C.POINTER(_P)]),
     "nsm_model_destroy": (_I, [_P]),
     "nsm_model_add_boundary_condition": (_I, [_P, _I, _I, _D, _I64, _P]),

@@ -165,6 +165,7 @@ struct nsm_ensemble {
   int*               d_conn = nullptr;
   uint32_t *         d_ptr = nullptr, *d_ent = nullptr;
   double*            d_slots = nullptr;      // FP64
+  double*            d_rjac = nullptr;       // FP64: reference inverse Jacobians, [E_mesh][8][10] per mesh
   double*            d_elem_mass = nullptr;  // setup
   double*            d_zero = nullptr;       // zero displacement of the largest mesh
   double*            d_wg_min = nullptr;
@@ -180,6 +181,7 @@ struct nsm_ensemble {
   bool               have_reaction = false;
   int                rx_coord      = -1;  // one coordinate for every mesh's reaction sum
   int64_t            N_total = 0, E_total = 0;
+  int64_t            slot_elems = 0;  // sum over samples of E_mesh rounded up to 8 (slot rows)
 
   // per run
   nsm::StepTimes* d_times = nullptr;
@@ -429,8 +431,8 @@ int build_device(nsm_ensemble* es)
     flags.insert(flags.end(), m.flags.begin(), m.flags.end());
     for (int64_t k = 0; k < 8 * m.E; k++) conn.push_back(m.dev_of[(size_t)m.conn[(size_t)k]]);
     // node -> (element, corner) incidences in ascending element order, as
-    // the slot offset corner * E + element; node_ptr values index the
-    // concatenated entries
+    // the slot index fp64_slot(element, corner) of the FP64 force slots and
+    // element masses; node_ptr values index the concatenated entries
     std::vector<uint32_t> cnt((size_t)m.N + 1, 0);
     const int*            cd = conn.data() + 8 * (size_t)m.elem_base;
     for (int64_t k = 0; k < 8 * m.E; k++) cnt[(size_t)cd[k] + 1]++;
@@ -441,7 +443,7 @@ int build_device(nsm_ensemble* es)
     {
       std::vector<uint32_t> fill(cnt.begin(), cnt.end() - 1);
       for (int64_t k = 0; k < 8 * m.E; k++)
-        ent[eb + fill[(size_t)cd[k]]++] = (uint32_t)((k & 7) * m.E + (k >> 3));
+        ent[eb + fill[(size_t)cd[k]]++] = (uint32_t)nsm::fp64_slot(k >> 3, (int)(k & 7));
     }
     nsm::EnsMesh d{};
     d.num_nodes = (int32_t)m.N;
@@ -486,7 +488,7 @@ int build_device(nsm_ensemble* es)
   std::vector<nsm::EnsSample> ds;
   std::vector<nsm::EnsRange>  elem_chunks, node_chunks, bnd_groups;
   std::vector<nsm::EnsTile>   tile_work;
-  int64_t                     node_base = 0, partial_base = 0, elem_prefix = 0, kg = 0;
+  int64_t                     node_base = 0, partial_base = 0, elem_prefix = 0, slot_elems = 0, kg = 0;
   for (int64_t s = 0; s < es->S(); s++) {
     const HostMesh& m = es->meshes[(size_t)es->samples[(size_t)s].mesh];
     nsm::EnsSample  d{};
@@ -496,7 +498,7 @@ int build_device(nsm_ensemble* es)
     d.partial_base = (int32_t)partial_base;
     d.bc_base      = (int32_t)(s * nsm::kEnsMaxBcCodes);
     d.kg_base      = (int32_t)kg;
-    d.slot_base    = 24 * elem_prefix;
+    d.slot_base    = 24 * slot_elems;  // [3][8 E8] per sample (E8: E_mesh rounded up to 8)
     ds.push_back(d);
     for (int b = 0; b < m.nb; b++)
       for (int64_t e = m.blk_off[(size_t)b]; e < m.blk_off[(size_t)b + 1]; e += kElemChunk)
@@ -514,13 +516,15 @@ int build_device(nsm_ensemble* es)
     node_base += m.N;
     partial_base += m.num_partials;
     elem_prefix += m.E;
+    slot_elems += nsm::round_up8(m.E);
     kg += m.nb;
   }
   if (node_base > (int64_t)INT32_MAX / 3 || partial_base > (int64_t)INT32_MAX / 3 ||
-      8 * elem_prefix > (int64_t)INT32_MAX)
+      8 * slot_elems > (int64_t)INT32_MAX)
     return fail(NSM_ERR_ARG, "ensemble too large for 32-bit row indices");
   es->N_total         = node_base;
   es->E_total         = elem_prefix;
+  es->slot_elems      = slot_elems;
   es->num_elem_chunks = (int)elem_chunks.size();
   es->num_node_chunks = (int)node_chunks.size();
   es->num_bnd_groups  = (int)bnd_groups.size();
@@ -584,7 +588,7 @@ int build_device(nsm_ensemble* es)
       (r = dalloc(B, &es->d_Gf, (size_t)NB, &tot)) ||
       (r = dalloc(B, &es->d_bc, (size_t)(S * nsm::kEnsMaxBcCodes), &tot)) ||
       (r = dalloc(B, &es->d_init, (size_t)(S * nsm::kEnsMaxBcCodes), &tot)) ||
-      (r = dalloc(B, &es->d_elem_mass, (size_t)(8 * es->E_total), &tot)) ||
+      (r = dalloc(B, &es->d_elem_mass, (size_t)(8 * es->slot_elems), &tot)) ||
       (r = dalloc(B, &es->d_zero, 3 * (size_t)max_nodes, &tot)) ||
       (r = dalloc(B, &es->d_wg_min, (size_t)max_elem_dt + 1, &tot)) || (r = dalloc(B, &es->d_status, 1, &tot)) ||
       (r = dalloc(B, &es->d_rx_partials, 2 * (size_t)std::max<int64_t>(es->rx_stride, 1), &tot)))
@@ -605,7 +609,14 @@ int build_device(nsm_ensemble* es)
   es->en.bc      = es->d_bc;
   es->en.init    = es->d_init;
   if (!mixed) {
-    if ((r = dalloc(B, &es->d_slots, (size_t)(24 * es->E_total), &tot))) return r;
+    if ((r = dalloc(B, &es->d_slots, (size_t)(24 * es->slot_elems), &tot)) ||
+        (r = dalloc(B, &es->d_rjac, conn.size() * 10, &tot)))
+      return r;
+    for (const HostMesh& m : es->meshes) {
+      nsm::launch_reference_jacobians(m.E, es->d_conn + 8 * (int64_t)m.elem_base, es->en.X + 3 * (int64_t)m.x_base,
+                                      es->d_rjac + 80 * (int64_t)m.elem_base, st);
+      ENS_HIP(hipGetLastError());
+    }
   } else {
     nsm::TileArgs& ta = es->ta;
     ta.tile_elems     = es->tile_elems;
@@ -689,8 +700,8 @@ int issue_run(nsm_ensemble* es, int64_t n, int64_t output_frequency)
                          es->d_bc, es_step, fold};
       nsm::launch_ens_element_tiled(w, es->ta, es->en.X, na, rx, es->d_status, es->stream);
     } else {
-      nsm::launch_ens_element_fp64(es->en, es->d_elem_chunks, es->num_elem_chunks, es->d_conn, es->d_K, es->d_G,
-                                   es->d_slots, es->d_status, es->stream);
+      nsm::launch_ens_element_fp64(es->en, es->d_elem_chunks, es->num_elem_chunks, es->d_conn, es->d_rjac, es->d_K,
+                                   es->d_G, es->d_slots, es->d_status, es->stream);
     }
     ENS_LAUNCHED();
     if ((r = time_end(es, ti))) return r;

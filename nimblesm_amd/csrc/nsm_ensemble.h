@@ -30,7 +30,7 @@ struct EnsSample {
   int32_t partial_base;  // mixed: first tile partial of the sample
   int32_t bc_base;       // first kinematic-condition entry
   int32_t kg_base;       // first block of the sample's moduli
-  int64_t slot_base;     // FP64: first element force slot ([3][8][E_mesh] per sample)
+  int64_t slot_base;     // FP64: first element force slot ([3][8 E8] per sample, E8: E_mesh rounded up to 8)
 };
 
 struct EnsMesh {
@@ -128,9 +128,11 @@ struct EnsFold {
 void launch_ens_reset(const EnsNodes& en, const EnsRange* chunks, int num_chunks, EnsStep es, hipStream_t s);
 // first half of step es.step (kick, BC, drift, BC)
 void launch_ens_prekick(const EnsNodes& en, const EnsRange* chunks, int num_chunks, EnsStep es, hipStream_t s);
-// FP64: element forces into each sample's slots
+// FP64: element forces into each sample's slots (rjac: each mesh's reference
+// inverse Jacobians, launch_reference_jacobians, at 80 elem_base)
 void launch_ens_element_fp64(const EnsNodes& en, const EnsRange* chunks, int num_chunks, const int* conn,
-                             const double* K, const double* G, double* slots, int* status, hipStream_t s);
+                             const double* rjac, const double* K, const double* G, double* slots, int* status,
+                             hipStream_t s);
 // FP64: CSR gather + node update, reaction partial per chunk at
 // rx.partials[chunk]; the first waves first fold a previous step's totals
 void launch_ens_node_fp64(const EnsNodes& en, const EnsRange* chunks, int num_chunks, const uint32_t* node_ptr,

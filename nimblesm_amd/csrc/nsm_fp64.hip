@@ -79,8 +79,51 @@ constexpr int kThreads = 8 * kEPW;  // 8 lanes per element
 
 using namespace fp64;
 
-// a = sum_j x_j (x) dN_j/dxi and b = sum_j X_j (x) dN_j/dxi at ipt q with
-// x = X + u (src/nimble_element.h:442-482).  xs/us are [8][3].
+// a = sum_j x_j (x) dN_j/dxi at ipt q with x = X + u, and b = sum_j X_j (x)
+// dN_j/dxi (src/nimble_element.h:442-482, which forms both in one loop; each
+// entry is its own chain of products and sums, so forming them apart gives
+// the same bits).  xs/us are [8][3].
+__device__ __forceinline__ void current_jacobian(int q, const double (*xs)[3], const double (*us)[3], double a[9])
+{
+  for (int i = 0; i < 9; i++) a[i] = 0.0;
+#pragma unroll
+  for (int j = 0; j < 8; j++) {
+    const double cc1 = xs[j][0] + us[j][0];
+    const double cc2 = xs[j][1] + us[j][1];
+    const double cc3 = xs[j][2] + us[j][2];
+    const double s1 = c_hex.dN[24 * q + 3 * j], s2 = c_hex.dN[24 * q + 3 * j + 1], s3 = c_hex.dN[24 * q + 3 * j + 2];
+    a[0] += cc1 * s1;
+    a[1] += cc1 * s2;
+    a[2] += cc1 * s3;
+    a[3] += cc2 * s1;
+    a[4] += cc2 * s2;
+    a[5] += cc2 * s3;
+    a[6] += cc3 * s1;
+    a[7] += cc3 * s2;
+    a[8] += cc3 * s3;
+  }
+}
+
+__device__ __forceinline__ void reference_jacobian(int q, const double (*xs)[3], double b[9])
+{
+  for (int i = 0; i < 9; i++) b[i] = 0.0;
+#pragma unroll
+  for (int j = 0; j < 8; j++) {
+    const double rc1 = xs[j][0], rc2 = xs[j][1], rc3 = xs[j][2];
+    const double s1 = c_hex.dN[24 * q + 3 * j], s2 = c_hex.dN[24 * q + 3 * j + 1], s3 = c_hex.dN[24 * q + 3 * j + 2];
+    b[0] += rc1 * s1;
+    b[1] += rc1 * s2;
+    b[2] += rc1 * s3;
+    b[3] += rc2 * s1;
+    b[4] += rc2 * s2;
+    b[5] += rc2 * s3;
+    b[6] += rc3 * s1;
+    b[7] += rc3 * s2;
+    b[8] += rc3 * s3;
+  }
+}
+
+// both in one pass over the nodes
 __device__ __forceinline__ void jacobians(int q, const double (*xs)[3], const double (*us)[3], double a[9],
                                           double b[9])
 {
@@ -88,9 +131,9 @@ __device__ __forceinline__ void jacobians(int q, const double (*xs)[3], const do
 #pragma unroll
   for (int j = 0; j < 8; j++) {
     const double rc1 = xs[j][0], rc2 = xs[j][1], rc3 = xs[j][2];
-    const double cc1 = xs[j][0] + us[j][0];
-    const double cc2 = xs[j][1] + us[j][1];
-    const double cc3 = xs[j][2] + us[j][2];
+    const double cc1 = rc1 + us[j][0];
+    const double cc2 = rc2 + us[j][1];
+    const double cc3 = rc3 + us[j][2];
     const double s1 = c_hex.dN[24 * q + 3 * j], s2 = c_hex.dN[24 * q + 3 * j + 1], s3 = c_hex.dN[24 * q + 3 * j + 2];
     a[0] += cc1 * s1;
     a[1] += cc1 * s2;
@@ -177,15 +220,33 @@ struct Lds {
 // kElementForcesFromStress (forces from the stress stored in sip, the
 // kernel after a block material interface's ComputeStress,
 // src/nimble_kokkos_model_data.cc:1229-1257).
-template <int kMode>
+//
+// kRefTable: each point's reference inverse Jacobian b^{-1} and det b are
+// read from rjac (k_reference_jacobians; b depends on X only), else formed
+// here as the reference does in every call (src/nimble_element.h:442-500);
+// the same operations either way, so the same bits.
+template <int kMode, bool kRefTable>
 __device__ __forceinline__ void element_fp64(Lds& lds, int64_t e, bool valid, double K, double G, int64_t E_total,
                                              const int* __restrict__ conn, const double* __restrict__ X,
-                                             const double* __restrict__ u, double* __restrict__ Fip,
-                                             double* __restrict__ sip, double* __restrict__ slots,
-                                             int* __restrict__ status)
+                                             const double* __restrict__ u, const double* __restrict__ rjac,
+                                             double* __restrict__ Fip, double* __restrict__ sip,
+                                             double* __restrict__ slots, int* __restrict__ status)
 {
   const int el = threadIdx.x >> 3;
   const int k  = threadIdx.x & 7;
+  double    bi[9], detb = 1.0;
+  if (kRefTable && kMode != kElementForcesFromStress && valid) {
+    const double2* r = (const double2*)(rjac + (e * 8 + k) * 10);
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+      const double2 v = r[i];
+      bi[2 * i]       = v.x;
+      bi[2 * i + 1]   = v.y;
+    }
+    const double2 v = r[4];
+    bi[8]           = v.x;
+    detb            = v.y;
+  }
   if (valid) {
     const int64_t nd = conn[e * 8 + k];
     lds.nodes.x[el][k][0] = X[3 * nd];
@@ -200,25 +261,31 @@ __device__ __forceinline__ void element_fp64(Lds& lds, int64_t e, bool valid, do
   double ainv[10], sig[6];
   int    bad = 0;
   if (valid) {
-    double a[9], b[9];
-    jacobians(k, lds.nodes.x[el], lds.nodes.u[el], a, b);
+    double a[9];
+    if (kRefTable || kMode == kElementForcesFromStress) {
+      current_jacobian(k, lds.nodes.x[el], lds.nodes.u[el], a);
+    } else {
+      double b[9];
+      jacobians(k, lds.nodes.x[el], lds.nodes.u[el], a, b);
+      detb = invert3x3(b, bi);
+    }
     if (kMode != kElementForcesFromStress) {
-      double bi[9], F[9];
-      bad |= invert3x3(b, bi) <= 0.0;
+      double F[9];
+      bad |= detb <= 0.0;
       def_grad(a, bi, F);
-      // element-fastest [comp][ipt][E] layout (no stores when Fip is null)
+      // groups of 8 elements (fp64_ip_index; no stores when Fip is null)
       if (Fip)
 #pragma unroll
-        for (int i = 0; i < 9; i++) Fip[(int64_t)(i * 8 + k) * E_total + e] = F[i];
+        for (int i = 0; i < 9; i++) Fip[fp64_ip_index(e, 9, i, k)] = F[i];
       if (kMode == kElementFull) {
         neohookean_stress(K, G, F, sig);
         if (sip)
 #pragma unroll
-          for (int i = 0; i < 6; i++) sip[(int64_t)(i * 8 + k) * E_total + e] = sig[i];
+          for (int i = 0; i < 6; i++) sip[fp64_ip_index(e, 6, i, k)] = sig[i];
       }
     } else {
 #pragma unroll
-      for (int i = 0; i < 6; i++) sig[i] = sip[(int64_t)(i * 8 + k) * E_total + e];
+      for (int i = 0; i < 6; i++) sig[i] = sip[fp64_ip_index(e, 6, i, k)];
     }
     // the force routine rebuilds a from the same x = X + u in the same order
     if (kMode != kElementDefGrad) {
@@ -241,33 +308,37 @@ __device__ __forceinline__ void element_fp64(Lds& lds, int64_t e, bool valid, do
   if (valid) {
     double f[3];
     node_force(k, lds.pts.ai[el], lds.pts.sg[el], f);
-    slots[(int64_t)k * E_total + e]       = f[0];  // [comp][node][E]
-    slots[(int64_t)(8 + k) * E_total + e]  = f[1];
-    slots[(int64_t)(16 + k) * E_total + e] = f[2];
+    const int64_t ss = 8 * round_up8(E_total), sl = fp64_slot(e, k);  // [comp][slot]
+    slots[sl]          = f[0];
+    slots[ss + sl]     = f[1];
+    slots[2 * ss + sl] = f[2];
   }
   if (bad) atomicOr(status, 1);
 }
 
-template <int kMode>
-__global__ __launch_bounds__(kThreads) void k_element_fp64(const BlockTable<double> bt, int64_t E_total,
-                                                           const int* __restrict__ conn,
-                                                           const double* __restrict__ X,
-                                                           const double* __restrict__ u,
-                                                           double* __restrict__ Fip, double* __restrict__ sip,
-                                                           double* __restrict__ slots, int* __restrict__ status)
+// The full evaluation is held to 5 waves per SIMD (at most 96 registers):
+// more would cost occupancy, fewer spill.
+template <int kMode, bool kRefTable>
+__global__ __launch_bounds__(kThreads, kMode == kElementFull && kRefTable ? 5 : 1) void k_element_fp64(
+    const BlockTable<double> bt, int64_t E_total, const int* __restrict__ conn, const double* __restrict__ X,
+    const double* __restrict__ u, const double* __restrict__ rjac, double* __restrict__ Fip,
+    double* __restrict__ sip, double* __restrict__ slots, int* __restrict__ status)
 {
   __shared__ Lds lds;
   const int     el    = threadIdx.x >> 3;
   const int64_t e     = bt.off[0] + (int64_t)xcd_tile(blockIdx.x, gridDim.x) * kEPW + el;
   const bool    valid = e < bt.off[bt.count];
   const int     blk   = valid ? block_of(bt, e) : 0;
-  element_fp64<kMode>(lds, e, valid, bt.K[blk], bt.G[blk], E_total, conn, X, u, Fip, sip, slots, status);
+  element_fp64<kMode, kRefTable>(lds, e, valid, bt.K[blk], bt.G[blk], E_total, conn, X, u, rjac, Fip, sip, slots,
+                                 status);
 }
 
 // Ensemble: workgroup g takes chunk chunks[g] (at most kEPW elements of one
-// block of one sample); the sample's slots are [3][8][E_mesh] at slot_base.
-__global__ __launch_bounds__(kThreads) void k_ens_element_fp64(EnsNodes en, const EnsRange* __restrict__ chunks,
+// block of one sample); the sample's slots are [3][8][E_mesh] at slot_base,
+// its mesh's reference Jacobians [E_mesh][8][10] at 80 elem_base.
+__global__ __launch_bounds__(kThreads, 5) void k_ens_element_fp64(EnsNodes en, const EnsRange* __restrict__ chunks,
                                                                const int* __restrict__ conn,
+                                                               const double* __restrict__ rjac,
                                                                const double* __restrict__ Kd,
                                                                const double* __restrict__ Gd,
                                                                double* __restrict__ slots, int* __restrict__ status)
@@ -278,10 +349,38 @@ __global__ __launch_bounds__(kThreads) void k_ens_element_fp64(EnsNodes en, cons
   const EnsMesh   ms = ens_load(en.meshes, (int64_t)sm.mesh);
   const int       el = threadIdx.x >> 3;
   const int       kb = sm.kg_base + ch.block;
-  element_fp64<kElementFull>(lds, (int64_t)ch.first + el, el < ch.count, ens_load(Kd, (int64_t)kb),
+  element_fp64<kElementFull, true>(lds, (int64_t)ch.first + el, el < ch.count, ens_load(Kd, (int64_t)kb),
                              ens_load(Gd, (int64_t)kb), ms.num_elem, conn + 8 * (int64_t)ms.elem_base,
-                             en.X + 3 * (int64_t)sm.x_base, en.u + 3 * (int64_t)sm.node_base, nullptr, nullptr,
-                             slots + sm.slot_base, status);
+                             en.X + 3 * (int64_t)sm.x_base, en.u + 3 * (int64_t)sm.node_base,
+                             rjac + 80 * (int64_t)ms.elem_base, nullptr, nullptr, slots + sm.slot_base, status);
+}
+
+// Reference inverse Jacobian and its determinant at every point of elements
+// [0, E) (Invert3x3 of b = sum_j X_j (x) dN_j/dxi, src/nimble_element.h:442-500):
+// out[(8 e + q) 10 + i] = b^{-1}[i] (row-major) for i < 9, det b for i = 9.
+// The same operations as the element kernels' would be, so the same bits.
+__global__ __launch_bounds__(kThreads) void k_reference_jacobians(int64_t E, const int* __restrict__ conn,
+                                                                  const double* __restrict__ X,
+                                                                  double* __restrict__ out)
+{
+  __shared__ double xs[kEPW][8][3];
+  const int     el = threadIdx.x >> 3, k = threadIdx.x & 7;
+  const int64_t e  = (int64_t)blockIdx.x * kEPW + el;
+  if (e < E) {
+    const int64_t nd = conn[e * 8 + k];
+    xs[el][k][0]     = X[3 * nd];
+    xs[el][k][1]     = X[3 * nd + 1];
+    xs[el][k][2]     = X[3 * nd + 2];
+  }
+  __syncthreads();
+  if (e < E) {
+    double b[9], bi[9];
+    reference_jacobian(k, xs[el], b);
+    const double det = invert3x3(b, bi);
+#pragma unroll
+    for (int i = 0; i < 9; i++) out[(e * 8 + k) * 10 + i] = bi[i];
+    out[(e * 8 + k) * 10 + 9] = det;
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -355,21 +454,21 @@ __global__ __launch_bounds__(256) void k_seam_stress(int64_t P, double K, double
 }
 
 // The same per point over num_elem x num_ipt points at arbitrary strides
-// (point p = e * num_ipt + q), so a block's STEP_NP1 views are used where
-// they lie: Kokkos LayoutLeft [E][8][9], LayoutRight, or a model's
-// [comp][ipt][E] arrays.
-__global__ __launch_bounds__(256) void k_seam_stress_strided(int64_t num_elem, int num_ipt, double K, double G,
-                                                             const double* __restrict__ F, IptStrides fs,
+// (point p = (e - first) * num_ipt + q), so a block's STEP_NP1 views are used
+// where they lie: Kokkos LayoutLeft [E][8][9], LayoutRight, or a model's
+// FP64 arrays (elements in groups of 8).
+__global__ __launch_bounds__(256) void k_seam_stress_strided(int64_t first, int64_t num_elem, int num_ipt, double K,
+                                                             double G, const double* __restrict__ F, IptStrides fs,
                                                              double* __restrict__ sig, IptStrides ss)
 {
   const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= num_elem * num_ipt) return;
-  const int64_t e = p / num_ipt, q = p % num_ipt;
+  const int64_t e = first + p / num_ipt, q = p % num_ipt;
   double        Fp[9], s[6];
-  const double* Fe = F + e * fs.elem + q * fs.ipt;
+  const double* Fe = F + (e >> 3) * fs.group + (e & 7) * fs.elem + q * fs.ipt;
   for (int i = 0; i < 9; i++) Fp[i] = Fe[i * fs.comp];
   neohookean_stress(K, G, Fp, s);
-  double* Se = sig + e * ss.elem + q * ss.ipt;
+  double* Se = sig + (e >> 3) * ss.group + (e & 7) * ss.elem + q * ss.ipt;
   for (int i = 0; i < 6; i++) Se[i * ss.comp] = s[i];
 }
 
@@ -450,8 +549,10 @@ __global__ __launch_bounds__(256) void k_seam_char_length(int64_t E, const doubl
 // ---------------------------------------------------------------------------
 // Model setup
 // ---------------------------------------------------------------------------
+// Element masses of elements [e_begin, e_end): element e's node j at
+// fp64_slot(e, j) (grouped) or j E_total + e.
 __global__ __launch_bounds__(kThreads) void k_element_mass(int64_t e_begin, int64_t e_end, int64_t E_total,
-                                                           const int* __restrict__ conn,
+                                                           int grouped, const int* __restrict__ conn,
                                                            const double* __restrict__ X, double rho,
                                                            double* __restrict__ elem_mass, int* __restrict__ status)
 {
@@ -473,11 +574,11 @@ __global__ __launch_bounds__(kThreads) void k_element_mass(int64_t e_begin, int6
     if (det[el][k] <= 0.0) atomicOr(status, 1);
   }
   __syncthreads();
-  if (e < e_end) lumped_row(k, rho, det[el], &elem_mass[(int64_t)k * E_total + e]);
+  if (e < e_end) lumped_row(k, rho, det[el], &elem_mass[grouped ? fp64_slot(e, k) : (int64_t)k * E_total + e]);
 }
 
 // Ensemble: element masses of chunk chunks[g] with its sample's block
-// density, into the sample's [8][E_mesh] rows at slot_base / 3.
+// density, element e's node j at slot_base / 3 + fp64_slot(e, j).
 __global__ __launch_bounds__(kThreads) void k_ens_element_mass(EnsNodes en, const EnsRange* __restrict__ chunks,
                                                                const int* __restrict__ conn,
                                                                const double* __restrict__ rho,
@@ -509,7 +610,7 @@ __global__ __launch_bounds__(kThreads) void k_ens_element_mass(EnsNodes en, cons
   __syncthreads();
   if (ok)
     lumped_row(k, ens_load(rho, (int64_t)(sm.kg_base + ch.block)), det[el],
-               &elem_mass[sm.slot_base / 3 + (int64_t)k * ms.num_elem + e]);
+               &elem_mass[sm.slot_base / 3 + fp64_slot(e, k)]);
 }
 
 // Minimum over the workgroup's elements of charlen / sound_speed on current
@@ -552,29 +653,43 @@ inline unsigned grid_of(int64_t n, int64_t per) { return (unsigned)((n + per - 1
 }  // namespace
 
 void launch_element_fp64(const BlockTable<double>& bt, int64_t E_total, const int* conn, const double* X,
-                         const double* u, double* Fip, double* sip, double* slots, int* status, hipStream_t s,
-                         int mode)
+                         const double* u, const double* rjac, double* Fip, double* sip, double* slots, int* status,
+                         hipStream_t s, int mode)
 {
   const int64_t n = bt.off[bt.count] - bt.off[0];
   if (n <= 0) return;
   const dim3 grid(grid_of(n, kEPW)), block(kThreads);
+#define NSM_FP64_LAUNCH(MODE)                                                                                  \
+  do {                                                                                                         \
+    if (rjac)                                                                                                  \
+      hipLaunchKernelGGL((k_element_fp64<MODE, true>), grid, block, 0, s, bt, E_total, conn, X, u, rjac, Fip, \
+                         sip, slots, status);                                                                  \
+    else                                                                                                       \
+      hipLaunchKernelGGL((k_element_fp64<MODE, false>), grid, block, 0, s, bt, E_total, conn, X, u, rjac, Fip, \
+                         sip, slots, status);                                                                  \
+  } while (0)
   if (mode == kElementDefGrad)
-    hipLaunchKernelGGL(k_element_fp64<kElementDefGrad>, grid, block, 0, s, bt, E_total, conn, X, u, Fip, sip, slots,
-                       status);
+    NSM_FP64_LAUNCH(kElementDefGrad);
   else if (mode == kElementForcesFromStress)
-    hipLaunchKernelGGL(k_element_fp64<kElementForcesFromStress>, grid, block, 0, s, bt, E_total, conn, X, u, Fip, sip,
-                       slots, status);
+    NSM_FP64_LAUNCH(kElementForcesFromStress);
   else
-    hipLaunchKernelGGL(k_element_fp64<kElementFull>, grid, block, 0, s, bt, E_total, conn, X, u, Fip, sip, slots,
-                       status);
+    NSM_FP64_LAUNCH(kElementFull);
+#undef NSM_FP64_LAUNCH
+}
+
+void launch_reference_jacobians(int64_t E, const int* conn, const double* X, double* out, hipStream_t s)
+{
+  if (E <= 0) return;
+  hipLaunchKernelGGL(k_reference_jacobians, dim3(grid_of(E, kEPW)), dim3(kThreads), 0, s, E, conn, X, out);
 }
 
 void launch_ens_element_fp64(const EnsNodes& en, const EnsRange* chunks, int num_chunks, const int* conn,
-                             const double* K, const double* G, double* slots, int* status, hipStream_t s)
+                             const double* rjac, const double* K, const double* G, double* slots, int* status,
+                             hipStream_t s)
 {
   if (num_chunks <= 0) return;
-  hipLaunchKernelGGL(k_ens_element_fp64, dim3((unsigned)num_chunks), dim3(kThreads), 0, s, en, chunks, conn, K, G,
-                     slots, status);
+  hipLaunchKernelGGL(k_ens_element_fp64, dim3((unsigned)num_chunks), dim3(kThreads), 0, s, en, chunks, conn, rjac, K,
+                     G, slots, status);
 }
 
 void launch_ens_element_mass(const EnsNodes& en, const EnsRange* chunks, int num_chunks, const int* conn,
@@ -598,13 +713,13 @@ void launch_seam_forces(int64_t E, const double* Xg, const double* ug, const dou
   hipLaunchKernelGGL(k_seam_forces, dim3(grid_of(E, kEPW)), dim3(kThreads), 0, s, E, Xg, ug, sig, f, status);
 }
 
-void launch_seam_stress_strided(int64_t num_elem, int num_ipt, double K, double G, const double* F, IptStrides fs,
-                                double* sig, IptStrides ss, hipStream_t s)
+void launch_seam_stress_strided(int64_t first, int64_t num_elem, int num_ipt, double K, double G, const double* F,
+                                IptStrides fs, double* sig, IptStrides ss, hipStream_t s)
 {
   const int64_t P = num_elem * num_ipt;
   if (P <= 0) return;
-  hipLaunchKernelGGL(k_seam_stress_strided, dim3(grid_of(P, 256)), dim3(256), 0, s, num_elem, num_ipt, K, G, F, fs,
-                     sig, ss);
+  hipLaunchKernelGGL(k_seam_stress_strided, dim3(grid_of(P, 256)), dim3(256), 0, s, first, num_elem, num_ipt, K, G,
+                     F, fs, sig, ss);
 }
 
 void launch_seam_stress(int64_t P, double K, double G, const double* F, double* sig, hipStream_t s)
@@ -625,12 +740,12 @@ void launch_seam_char_length(int64_t E, const double* xg, double* out, hipStream
   hipLaunchKernelGGL(k_seam_char_length, dim3(grid_of(E, 256)), dim3(256), 0, s, E, xg, out);
 }
 
-void launch_element_mass(int64_t e_begin, int64_t e_end, int64_t E_total, const int* conn, const double* X,
-                         double rho, double* elem_mass, int* status, hipStream_t s)
+void launch_element_mass(int64_t e_begin, int64_t e_end, int64_t E_total, bool grouped, const int* conn,
+                         const double* X, double rho, double* elem_mass, int* status, hipStream_t s)
 {
   if (e_end <= e_begin) return;
   hipLaunchKernelGGL(k_element_mass, dim3(grid_of(e_end - e_begin, kEPW)), dim3(kThreads), 0, s, e_begin, e_end,
-                     E_total, conn, X, rho, elem_mass, status);
+                     E_total, (int)grouped, conn, X, rho, elem_mass, status);
 }
 
 int launch_element_dt(int64_t e_begin, int64_t e_end, const int* conn, const double* X, const double* u,

@@ -99,13 +99,34 @@ __device__ __forceinline__ int block_of(const BlockTable<Real>& bt, int64_t e)
   return b;
 }
 
+// ---- FP64 element-data layout ----
+// The FP64 element kernel's wavefront holds 8 consecutive elements, so its
+// element data is stored in groups of 8 elements, one group contiguous:
+// value (c, q) of element e (nc components, 8 points) at
+//   64 nc (e / 8) + 64 c + 8 q + e % 8,
+// and the force slot of element e's node j (the node gather's index; three
+// components at slot_stride = 8 E8 apart, E8 = E rounded up to 8) at
+//   64 (e / 8) + 8 j + e % 8.
+// A store of one component then covers 512 contiguous bytes per wavefront.
+__host__ __device__ __forceinline__ int64_t fp64_ip_index(int64_t e, int nc, int c, int q)
+{
+  return 64 * (int64_t)nc * (e >> 3) + 64 * c + 8 * q + (e & 7);
+}
+__host__ __device__ __forceinline__ int64_t fp64_slot(int64_t e, int j) { return 64 * (e >> 3) + 8 * j + (e & 7); }
+__host__ __device__ __forceinline__ int64_t round_up8(int64_t n) { return (n + 7) & ~(int64_t)7; }
+
 // ---- element kernels over the elements of a block table ----
 // FP64 element kernel modes: F, stress and forces; F only; forces from the
 // stress already in sip
 constexpr int kElementFull = 0, kElementDefGrad = 1, kElementForcesFromStress = 2;
+// rjac: the reference inverse Jacobians of launch_reference_jacobians
 void launch_element_fp64(const BlockTable<double>& bt, int64_t E_total, const int* conn, const double* X,
-                         const double* u, double* Fip, double* sip, double* slots, int* status, hipStream_t s,
-                         int mode = kElementFull);
+                         const double* u, const double* rjac, double* Fip, double* sip, double* slots, int* status,
+                         hipStream_t s, int mode = kElementFull);
+// b^{-1} (row-major) and det b of the reference Jacobian b at every point of
+// elements [0, E): out[(8 e + q) 10 + i], i < 9 the inverse, i = 9 det b
+// ([E][8][10] doubles).  A function of X alone, formed once per mesh.
+void launch_reference_jacobians(int64_t E, const int* conn, const double* X, double* out, hipStream_t s);
 void launch_element_mixed(const BlockTable<float>& bt, int64_t E_total, const int* conn, const double* X,
                           const double* u, float* Fip, float* sip, float* slots, int* status, hipStream_t s);
 
@@ -114,20 +135,22 @@ void launch_seam_defgrad(int64_t E, const double* Xg, const double* ug, double* 
 void launch_seam_forces(int64_t E, const double* Xg, const double* ug, const double* sig, double* f, int* status,
                         hipStream_t s);
 void launch_seam_stress(int64_t P, double K, double G, const double* F, double* sig, hipStream_t s);
-// stress of num_elem x num_ipt points: F value (e, q, c) at F[e fe + q fq + c fc],
-// sigma value at sig[e se + q sq + c sc]
+// stress of the points of elements e in [first, first + num_elem): F value
+// (e, q, c) at F[(e / 8) fg + (e % 8) fe + q fq + c fc], sigma value at the
+// same form with the sigma strides (plain strides: g = 8 e, first = 0)
 struct IptStrides {
-  int64_t elem, ipt, comp;
+  int64_t group, elem, ipt, comp;
 };
-void launch_seam_stress_strided(int64_t num_elem, int num_ipt, double K, double G, const double* F, IptStrides fs,
-                                double* sig, IptStrides ss, hipStream_t s);
+void launch_seam_stress_strided(int64_t first, int64_t num_elem, int num_ipt, double K, double G, const double* F,
+                                IptStrides fs, double* sig, IptStrides ss, hipStream_t s);
 void launch_seam_lumped_mass(int64_t E, double rho, const double* Xg, double* m, int* status, hipStream_t s);
 void launch_seam_char_length(int64_t E, const double* xg, double* out, hipStream_t s);
 
 // ---- model setup ----
-// element masses in slot layout: elem_mass[k*E_total + e]
-void launch_element_mass(int64_t e_begin, int64_t e_end, int64_t E_total, const int* conn, const double* X,
-                         double rho, double* elem_mass, int* status, hipStream_t s);
+// element masses in the node gather's slot order: element e's node k at
+// elem_mass[fp64_slot(e, k)] (grouped) or elem_mass[k*E_total + e]
+void launch_element_mass(int64_t e_begin, int64_t e_end, int64_t E_total, bool grouped, const int* conn,
+                         const double* X, double rho, double* elem_mass, int* status, hipStream_t s);
 // per-workgroup minimum of charlen/c over [e_begin, e_end) into wg_min[blockIdx]
 int  launch_element_dt(int64_t e_begin, int64_t e_end, const int* conn, const double* X, const double* u,
                        double sound_speed, double* wg_min, hipStream_t s);

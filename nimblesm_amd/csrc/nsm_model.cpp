@@ -146,6 +146,7 @@ struct nsm_model {
   void*     Fip = nullptr;       // double or float [9][8][E] (component, ipt, element); tiled: per tile
   void*     sip = nullptr;       // double or float [6][8][E]; tiled: per tile (see ip_index)
   void*     slots = nullptr;     // double or float [3][8][E] element force slots
+  double*   rjac = nullptr;      // FP64: reference inverse Jacobians and det, [E][8][10] (small meshes, see use_reference_table)
   double*   elem_tmp = nullptr;  // [8][E] setup scratch (element masses)
   double*   wg_min = nullptr;
   int64_t   wg_min_len = 0;
@@ -166,6 +167,10 @@ struct nsm_model {
   // tile-local assembly (mixed precision): element tiles, device copies of
   // their arrays; integration-point arrays are stored in tile element order
   bool                 tiled = false;
+  // element masses (and FP64 force slots) in the grouped slot order
+  // fp64_slot(e, j): FP64 and tiled models; untiled mixed ones keep j E + e,
+  // the order of their element kernel's FP32 slots
+  bool grouped() const { return precision == NSM_FP64 || tiled; }
   nsm::TileSet         tiles;
   std::vector<int>     blk_tile_off;  // first tile of each block, [num_blocks + 1]
   nsm::TileArgs        tile_args{};
@@ -202,7 +207,7 @@ struct nsm_model {
   {
     nsm::NodeArrays na;
     na.num_nodes   = N;
-    na.slot_stride = 8 * E;
+    na.slot_stride = grouped() ? 8 * nsm::round_up8(E) : 8 * E;
     na.node_ptr    = node_ptr;
     na.node_ent  = node_ent;
     na.mass      = m;
@@ -243,13 +248,20 @@ inline int64_t dev_node(const nsm_model* md, int64_t n) { return md->tiled ? md-
 inline int64_t mesh_node(const nsm_model* md, int64_t i) { return md->tiled ? md->tiles.node_old[(size_t)i] : i; }
 
 // Calls f(device index, device element, component, point) for every value of
-// an integration-point field with nc components: element-fastest rows
-// [c][q][E] in untiled models; in tiled ones each tile t (device elements
-// [e0, e0 + n)) owns [8 nc e0, 8 nc (e0 + n)) with rows (4 c + q/2) of length
-// 2n, element l's points q and q+1 (q even) side by side.
+// an integration-point field with nc components: groups of 8 elements
+// (fp64_ip_index) in FP64 models; element-fastest rows [c][q][E] in untiled
+// mixed ones; in tiled ones each tile t (device elements [e0, e0 + n)) owns
+// [8 nc e0, 8 nc (e0 + n)) with rows (4 c + q/2) of length 2n, element l's
+// points q and q+1 (q even) side by side.
 template <typename Fn>
 void for_each_ip(const nsm_model* md, int nc, Fn&& fn)
 {
+  if (md->precision == NSM_FP64) {
+    for (int64_t e = 0; e < md->E; e++)
+      for (int c = 0; c < nc; c++)
+        for (int q = 0; q < 8; q++) fn(nsm::fp64_ip_index(e, nc, c, q), e, c, q);
+    return;
+  }
   if (!md->tiled) {
     for (int c = 0; c < nc; c++)
       for (int q = 0; q < 8; q++)
@@ -476,7 +488,7 @@ int launch_elements(nsm_model* md, int sample, int fp64_mode = nsm::kElementFull
         bt.K[b] = md->bulk(sample, b0 + b);
         bt.G[b] = md->shear(sample, b0 + b);
       }
-      nsm::launch_element_fp64(bt, md->E, md->conn, md->X, u, (double*)ip_out(md, md->Fip, sample),
+      nsm::launch_element_fp64(bt, md->E, md->conn, md->X, u, md->rjac, (double*)ip_out(md, md->Fip, sample),
                                (double*)ip_out(md, md->sip, sample), (double*)md->slots, md->status, md->stream,
                                fp64_mode);
     } else {
@@ -839,6 +851,21 @@ int model_create(int64_t num_nodes, const double* reference_coordinates, int num
                  const int64_t* block_num_elem, const int* conn, const double* density, const double* bulk_modulus,
                  const double* shear_modulus, int precision, int tile_override, nsm_model** out);
 
+// Whether an FP64 model of E elements reads its reference inverse Jacobians
+// from a table formed once (80 doubles per element) instead of forming them
+// in every element launch: NSM_FP64_REF_TABLE=0/1 when set, else when the
+// table takes at most kRefTableBytes.  Reading the table saves 13% of the
+// element kernel's instructions, but it is a further 640 bytes per element
+// and launch: it pays while the table stays in the device's caches between
+// launches, and costs time once it streams from HBM.
+constexpr int64_t kRefTableBytes = 96ll << 20;
+bool use_reference_table(int64_t E)
+{
+  const char* v = std::getenv("NSM_FP64_REF_TABLE");
+  if (v && *v) return std::strcmp(v, "0") != 0;
+  return E > 0 && 640 * E <= kRefTableBytes;
+}
+
 // Element-tile size of a mixed model: NSM_TILE_ELEMS when it names one;
 // 512 when that already gives every CU two tiles or more; otherwise the
 // autotuner's choice from a few timed steps of a model built with each
@@ -921,7 +948,7 @@ int model_create(int64_t num_nodes, const double* reference_coordinates, int num
   }
   const int64_t E = off.back();
   if (E > 0 && !conn) return fail(NSM_ERR_ARG, "conn is null");
-  if (8 * E >= (int64_t)UINT32_MAX) return fail(NSM_ERR_ARG, "too many elements for 32-bit CSR entries");
+  if (8 * nsm::round_up8(E) >= (int64_t)UINT32_MAX) return fail(NSM_ERR_ARG, "too many elements for 32-bit CSR entries");
   for (int64_t k = 0; k < 8 * E; k++)
     if (conn[k] < 0 || conn[k] >= num_nodes)
       return fail(NSM_ERR_ARG, "connectivity entry %lld = %d out of range", (long long)k, conn[k]);
@@ -986,20 +1013,26 @@ int model_create(int64_t num_nodes, const double* reference_coordinates, int num
   }
 
   // node -> (element e, local node j) incidence in ascending element order,
-  // stored as the slot offset j*E + e of the element-fastest force slots
+  // stored as the slot index of (e, j): fp64_slot(e, j) in grouped models,
+  // else j*E + e (the element-fastest FP32 slots of the untiled mixed kernel)
   std::vector<uint32_t> ptr((size_t)num_nodes + 1, 0), ent((size_t)(8 * E));
   for (int64_t k = 0; k < 8 * E; k++) ptr[(size_t)cd[k] + 1]++;
   for (int64_t i = 0; i < num_nodes; i++) ptr[(size_t)i + 1] += ptr[(size_t)i];
   {
     std::vector<uint32_t> fill(ptr.begin(), ptr.end() - 1);
-    for (int64_t k = 0; k < 8 * E; k++) ent[fill[(size_t)cd[k]]++] = (uint32_t)((k & 7) * E + (k >> 3));
+    const bool            grouped = md->grouped();
+    for (int64_t k = 0; k < 8 * E; k++)
+      ent[fill[(size_t)cd[k]]++] =
+          (uint32_t)(grouped ? nsm::fp64_slot(k >> 3, (int)(k & 7)) : (k & 7) * E + (k >> 3));
   }
 
   const size_t N3 = 3 * (size_t)num_nodes;
   const size_t ipw = precision == NSM_FP64 ? sizeof(double) : sizeof(float);
   size_t&      tot = md->device_bytes;
   int          st  = 0;
-  const size_t num_slots    = md->tiled ? 0 : (size_t)(24 * E);
+  // FP64 element data in groups of 8 elements (fp64_ip_index, fp64_slot)
+  const size_t E_ip         = precision == NSM_FP64 ? (size_t)nsm::round_up8(E) : (size_t)E;
+  const size_t num_slots    = md->tiled ? 0 : 24 * E_ip;
   const size_t num_partials = md->tiled ? (size_t)md->tiles.num_tiles +
                                               nsm::boundary_grid((int)md->tiles.bnd_nodes.size())
                                         : (size_t)nsm::node_grid(num_nodes);
@@ -1008,9 +1041,10 @@ int model_create(int64_t num_nodes, const double* reference_coordinates, int num
       (st = dalloc(&md->a, N3, &tot)) || (st = dalloc(&md->f, N3, &tot)) ||
       (st = dalloc(&md->m, (size_t)num_nodes, &tot)) || (st = dalloc(&md->conn, (size_t)(8 * E), &tot)) ||
       (st = dalloc(&md->node_ptr, ptr.size(), &tot)) || (st = dalloc(&md->node_ent, ent.size(), &tot)) ||
-      (st = dalloc((char**)&md->Fip, (size_t)(72 * E) * ipw, &tot)) ||
-      (st = dalloc((char**)&md->sip, (size_t)(48 * E) * ipw, &tot)) ||
+      (st = dalloc((char**)&md->Fip, 72 * E_ip * ipw, &tot)) ||
+      (st = dalloc((char**)&md->sip, 48 * E_ip * ipw, &tot)) ||
       (st = dalloc((char**)&md->slots, num_slots * ipw, &tot)) ||
+      (st = dalloc(&md->rjac, precision == NSM_FP64 && use_reference_table(E) ? (size_t)(80 * E) : 0, &tot)) ||
       (st = dalloc(&md->status, 1, &tot)) || (st = dalloc(&md->flags, (size_t)num_nodes, &tot)) ||
       (st = dalloc(&md->reaction_partials, 2 * std::max<size_t>(num_partials, 1), &tot)) ||
       (md->tiled && (st = setup_tiles(md))))
@@ -1035,11 +1069,15 @@ int model_create(int64_t num_nodes, const double* reference_coordinates, int num
   NSM_HIP_C(hipMemsetAsync(md->m, 0, (size_t)num_nodes * sizeof(double), s));
   NSM_HIP_C(hipMemsetAsync(md->status, 0, sizeof(int), s));
   NSM_HIP_C(hipMemsetAsync(md->flags, 0, (size_t)num_nodes, s));
+  if (md->rjac) {
+    nsm::launch_reference_jacobians(E, md->conn, md->X, md->rjac, s);
+    NSM_HIP_C(hipGetLastError());
+  }
   if (E > 0) {
     // STEP_NP1 deformation gradient starts at the identity, stress at zero
     // (src/nimble_kokkos_model_data.cc:921-924); the diagonal components XX,
     // YY, ZZ are components 0-2
-    std::vector<char> ident((size_t)(72 * E) * ipw, 0);
+    std::vector<char> ident(72 * E_ip * ipw, 0);
     for_each_ip(md, 9, [&](int64_t p, int64_t, int c, int) {
       if (c >= 3) return;
       if (precision == NSM_FP64)
@@ -1048,7 +1086,7 @@ int model_create(int64_t num_nodes, const double* reference_coordinates, int num
         ((float*)ident.data())[p] = 1.0f;
     });
     NSM_HIP_C(hipMemcpy(md->Fip, ident.data(), ident.size(), hipMemcpyHostToDevice));
-    NSM_HIP_C(hipMemsetAsync(md->sip, 0, (size_t)(48 * E) * ipw, s));
+    NSM_HIP_C(hipMemsetAsync(md->sip, 0, 48 * E_ip * ipw, s));
   }
   NSM_HIP_C(hipStreamSynchronize(s));
 #undef NSM_HIP_C
@@ -1077,6 +1115,7 @@ int nsm_model_destroy(nsm_model* md)
   free_dev(md->Fip);
   free_dev(md->sip);
   free_dev(md->slots);
+  free_dev(md->rjac);
   free_dev(md->elem_tmp);
   free_dev(md->wg_min);
   free_dev(md->status);
@@ -1158,11 +1197,12 @@ int nsm_model_compute_lumped_mass(nsm_model* md)
   int st = ensure_device(md);
   if (st) return st;
   size_t tmp = 0;
-  if (!md->elem_tmp && (st = dalloc(&md->elem_tmp, std::max<size_t>((size_t)(8 * md->E), 1), &tmp))) return st;
+  if (!md->elem_tmp && (st = dalloc(&md->elem_tmp, std::max<size_t>((size_t)(8 * nsm::round_up8(md->E)), 1), &tmp)))
+    return st;
   md->device_bytes += tmp;
   for (int b = 0; b < md->num_blocks; b++) {
-    nsm::launch_element_mass(md->blk_off[b], md->blk_off[b + 1], md->E, md->conn, md->X, md->rho[b], md->elem_tmp,
-                             md->status, md->stream);
+    nsm::launch_element_mass(md->blk_off[b], md->blk_off[b + 1], md->E, md->grouped(), md->conn, md->X, md->rho[b],
+                             md->elem_tmp, md->status, md->stream);
     NSM_LAUNCHED();
   }
   nsm::launch_mass_gather(md->N, md->node_ptr, md->node_ent, md->elem_tmp, md->m, md->stream);
@@ -1262,9 +1302,28 @@ int nsm_neohookean_get_stress_strided(int64_t num_elem, int num_ipt, double bulk
   if (num_elem < 0 || num_ipt < 0) return fail(NSM_ERR_ARG, "negative size");
   if (num_elem == 0 || num_ipt == 0) return NSM_OK;
   if (!F_np1 || !sigma_np1) return fail(NSM_ERR_ARG, "null pointer");
-  nsm::launch_seam_stress_strided(num_elem, num_ipt, bulk_modulus, shear_modulus, F_np1,
-                                  nsm::IptStrides{F_elem_stride, F_ipt_stride, F_comp_stride}, sigma_np1,
-                                  nsm::IptStrides{sigma_elem_stride, sigma_ipt_stride, sigma_comp_stride},
+  // plain strides: a group of 8 elements is 8 element strides
+  nsm::launch_seam_stress_strided(0, num_elem, num_ipt, bulk_modulus, shear_modulus, F_np1,
+                                  nsm::IptStrides{8 * F_elem_stride, F_elem_stride, F_ipt_stride, F_comp_stride},
+                                  sigma_np1,
+                                  nsm::IptStrides{8 * sigma_elem_stride, sigma_elem_stride, sigma_ipt_stride,
+                                                  sigma_comp_stride},
+                                  (hipStream_t)stream);
+  NSM_LAUNCHED();
+  return NSM_OK;
+}
+
+int nsm_neohookean_get_stress_grouped(int64_t first_elem, int64_t num_elem, int num_ipt, double bulk_modulus,
+                                      double shear_modulus, const double* F_np1, const int64_t* F_strides,
+                                      double* sigma_np1, const int64_t* sigma_strides, void* stream)
+{
+  if (first_elem < 0 || num_elem < 0 || num_ipt < 0) return fail(NSM_ERR_ARG, "negative size");
+  if (num_elem == 0 || num_ipt == 0) return NSM_OK;
+  if (!F_np1 || !sigma_np1 || !F_strides || !sigma_strides) return fail(NSM_ERR_ARG, "null pointer");
+  nsm::launch_seam_stress_strided(first_elem, num_elem, num_ipt, bulk_modulus, shear_modulus, F_np1,
+                                  nsm::IptStrides{F_strides[0], F_strides[1], F_strides[2], F_strides[3]}, sigma_np1,
+                                  nsm::IptStrides{sigma_strides[0], sigma_strides[1], sigma_strides[2],
+                                                  sigma_strides[3]},
                                   (hipStream_t)stream);
   NSM_LAUNCHED();
   return NSM_OK;
@@ -1467,8 +1526,9 @@ int nsm_model_get_field(nsm_model* md, int field, double* out)
       const int    nc  = field == NSM_FIELD_DEFORMATION_GRADIENT ? 9 : 6;
       const void*  p   = field == NSM_FIELD_DEFORMATION_GRADIENT ? md->Fip : md->sip;
       const size_t E   = (size_t)md->E;
-      cnt              = (size_t)nc * 8 * E;
-      if (cnt == 0) return NSM_OK;
+      if (E == 0) return NSM_OK;
+      // device values: FP64 groups of 8 elements cover E rounded up to 8
+      cnt = (size_t)nc * 8 * (md->precision == NSM_FP64 ? (size_t)nsm::round_up8(md->E) : E);
       std::vector<double> tmp(cnt);
       if (md->precision == NSM_FP64) {
         NSM_HIP(hipMemcpyAsync(tmp.data(), p, cnt * sizeof(double), hipMemcpyDeviceToHost, md->stream));

@@ -347,7 +347,7 @@ __global__ __launch_bounds__(kNodeThreads) void k_ens_node_fp64(EnsNodes en, con
   if (n.valid) {
     double f[3];
     gather_force(node_ptr + ms.ptr_base + ch.first + threadIdx.x, node_ent, slots + sm.slot_base,
-                 8 * (int64_t)ms.num_elem, f);
+                 8 * round_up8(ms.num_elem), f);
     const int flags = en.flags[n.xrow];
     if (rx.coord >= 0 && (flags & kReactionFlag)) {
       member  = 1;

@@ -100,11 +100,24 @@ class NeohookeanMaterial:
 
     def GetStressStrided(self, num_elem, num_ipt, F_ptr, F_strides, sigma_ptr, sigma_strides, stream=None):
         """The stress of num_elem x num_ipt points stored at (element, point,
-        component) strides, e.g. one block of a model's STEP_NP1 arrays
+        component) strides, e.g. Kokkos LayoutLeft / LayoutRight views
         (device addresses)."""
         check(lib().nsm_neohookean_get_stress_strided(int(num_elem), int(num_ipt), self.bulk_modulus,
                                                       self.shear_modulus, F_ptr, *map(int, F_strides), sigma_ptr,
                                                       *map(int, sigma_strides), _stream(stream)))
+
+    def GetStressGrouped(self, first_elem, num_elem, num_ipt, F_ptr, F_strides, sigma_ptr, sigma_strides,
+                         stream=None):
+        """The same over arrays whose elements are stored in groups of 8, e.g.
+        one block of an FP64 model's STEP_NP1 arrays: strides (group,
+        element, point, component), elements [first_elem, first_elem +
+        num_elem) (device addresses)."""
+        fs = np.ascontiguousarray(F_strides, np.int64)
+        ss = np.ascontiguousarray(sigma_strides, np.int64)
+        assert fs.size == 4 and ss.size == 4
+        check(lib().nsm_neohookean_get_stress_grouped(int(first_elem), int(num_elem), int(num_ipt), self.bulk_modulus,
+                                                      self.shear_modulus, F_ptr, fs.ctypes.data, sigma_ptr,
+                                                      ss.ctypes.data, _stream(stream)))
 
 
 @dataclass

@@ -87,7 +87,12 @@ def test_reference_decks_fp64(hip_device, name):
         assert err == 0.0, (name, k, err)
 
 
-def test_cube16_fp64_1000_steps(hip_device):
+@pytest.mark.parametrize("ref_table", ["1", "0"])
+def test_cube16_fp64_1000_steps(hip_device, monkeypatch, ref_table):
+    """1000 FP64 steps bit-identical to the oracle, with the reference inverse
+    Jacobians read from the table formed at setup and formed in every launch
+    (NSM_FP64_REF_TABLE; small meshes use the table by default)."""
+    monkeypatch.setenv("NSM_FP64_REF_TABLE", ref_table)
     prob = N.workload("cube16_fp64")
     steps = prob.num_steps
     m, integ = run_gpu_problem(prob, steps, "fp64")
@@ -103,8 +108,13 @@ def test_cube16_fp64_1000_steps(hip_device):
     assert np.abs(R).max() > 0
 
 
-def test_one_step_internal_force_two_blocks(hip_device):
-    prob = N.CubeProblem(n=8, materials=[N.problems.STEEL, N.problems.ALUMINIUM], num_steps=1, seed=77)
+@pytest.mark.parametrize("n,ref_table", [(8, "1"), (9, "1"), (9, "0")])
+def test_one_step_internal_force_two_blocks(hip_device, monkeypatch, n, ref_table):
+    """Forces, F and sigma of one evaluation bit-identical to the oracle; n = 9
+    (729 elements) ends the FP64 element groups of 8 with a partial one and
+    starts the second block inside a group."""
+    monkeypatch.setenv("NSM_FP64_REF_TABLE", ref_table)
+    prob = N.CubeProblem(n=n, materials=[N.problems.STEEL, N.problems.ALUMINIUM], num_steps=1, seed=77)
     m = prob.make_model("fp64")
     rng = np.random.default_rng(2)
     u = rng.normal(0, 1e-4, (prob.num_nodes, 3))
@@ -351,11 +361,14 @@ def test_sampled_reaction_history(hip_device):
         m.close()
 
 
-@pytest.mark.parametrize("mode", ["fp64", "tiles", "slots"])
+@pytest.mark.parametrize("mode", ["fp64", "fp64-no-table", "tiles", "slots"])
 def test_inverted_element_in_model_reports_error_and_recovers(hip_device, monkeypatch, mode):
     """A displacement that turns elements inside out makes the model's force
     evaluation fail with NSM_ERR_NEG_JACOBIAN (the reference's Invert3x3
     check); the status is cleared, so a valid displacement then works."""
+    if mode == "fp64-no-table":
+        monkeypatch.setenv("NSM_FP64_REF_TABLE", "0")
+        mode = "fp64"
     if mode != "fp64":
         monkeypatch.setenv("NSM_MIXED_ASSEMBLY", mode)
     prob = N.CubeProblem(n=4, materials=[N.problems.STEEL], num_steps=1)

@@ -102,7 +102,9 @@ def test_inverted_element_reports_error(hip_device):
 
 def test_stress_strided_layouts(hip_device, elems):
     """The strided stress seam on LayoutRight [E][8][9] and LayoutLeft
-    (element-fastest) arrays gives the oracle's stress bit for bit."""
+    (element-fastest) arrays, and the grouped one on arrays in groups of 8
+    elements from an element that does not start a group, give the oracle's
+    stress bit for bit."""
     import torch
 
     X, u = elems
@@ -119,14 +121,26 @@ def test_stress_strided_layouts(hip_device, elems):
     Fl = _dev(np.ascontiguousarray(Fo.transpose(2, 1, 0)), hip_device)
     Sl = torch.zeros((6, 8, E), dtype=torch.float64, device=hip_device)
     mat.GetStressStrided(E, 8, Fl.data_ptr(), (1, E, 8 * E), Sl.data_ptr(), (1, E, 8 * E))
+    # groups of 8 elements (an FP64 model's layout), elements [3, E): value
+    # (e, q, c) at 64 nc (e / 8) + 64 c + 8 q + e % 8
+    E8 = (E + 7) // 8 * 8
+    Fg_h = np.zeros((E8 // 8, 9, 8, 8))
+    Fg_h[np.arange(E) // 8, :, :, np.arange(E) % 8] = Fo.transpose(0, 2, 1)
+    Fg = _dev(Fg_h, hip_device)
+    Sg = torch.zeros((E8 // 8, 6, 8, 8), dtype=torch.float64, device=hip_device)
+    mat.GetStressGrouped(3, E - 3, 8, Fg.data_ptr(), (576, 1, 8, 64), Sg.data_ptr(), (384, 1, 8, 64))
     torch.cuda.synchronize()
     assert np.array_equal(Sr.cpu().numpy(), ref)
     assert np.array_equal(Sl.cpu().numpy().transpose(2, 1, 0), ref)
+    Sg_h = Sg.cpu().numpy()
+    e = np.arange(3, E)
+    assert np.array_equal(Sg_h[e // 8, :, :, e % 8].transpose(0, 2, 1), ref[3:])
+    assert not Sg_h[0, :, :, :3].any()  # elements before first_elem untouched
 
 
 def test_split_force_evaluation_matches_fused(hip_device):
     """ComputeInternalForce split around a stress plug-in (F_np1 on the
-    device, the Neo-Hookean stress of each block through the strided seam on
+    device, the Neo-Hookean stress of each block through the grouped seam on
     the model's own arrays and stream, then the forces from that stress) is
     bit-identical to the fused evaluation, and to the oracle."""
     from tests.helpers import oracle_model_from_problem
@@ -147,7 +161,7 @@ def test_split_force_evaluation_matches_fused(hip_device):
     E = split.num_elem
     for b, blk in enumerate(split.blocks):
         e0, e1 = int(split.block_offsets[b]), int(split.block_offsets[b + 1])
-        blk.material.GetStressStrided(e1 - e0, 8, Fp + 8 * e0, (1, E, 8 * E), Sp + 8 * e0, (1, E, 8 * E),
+        blk.material.GetStressGrouped(e0, e1 - e0, 8, Fp, (576, 1, 8, 64), Sp, (384, 1, 8, 64),
                                       stream=split.Stream())
     split.ComputeInternalForceFromStress()
     om = oracle_model_from_problem(prob)
