@@ -567,6 +567,40 @@ int build_device(nsm_ensemble* es)
       fold_ptr.push_back((int)fold_idx.size());
     }
   }
+  // Work of the samples of one mesh side by side: element chunks by (mesh,
+  // block, chunk, sample), tile work items by (mesh, tile, sample).  The
+  // workgroups resident at one time then share the chunk's or tile's
+  // connectivity, coordinates and (FP64) reference Jacobians in L2.  Each
+  // item still writes only its own sample's rows, so the order changes no
+  // result; the reaction folds follow the tile items to their new positions.
+  // NSM_ENS_CHUNK_ORDER=sample keeps the sample-major order, for comparison.
+  {
+    const char* ord          = std::getenv("NSM_ENS_CHUNK_ORDER");
+    const bool  sample_major = ord && std::strcmp(ord, "sample") == 0;
+    if (!sample_major) {
+      auto mesh_of = [&](int32_t s) { return es->samples[(size_t)s].mesh; };
+      std::stable_sort(elem_chunks.begin(), elem_chunks.end(), [&](const nsm::EnsRange& x, const nsm::EnsRange& y) {
+        if (mesh_of(x.s) != mesh_of(y.s)) return mesh_of(x.s) < mesh_of(y.s);
+        return x.first < y.first;  // blocks are ascending element ranges
+      });
+      std::vector<int> perm(tile_work.size());
+      for (size_t i = 0; i < perm.size(); i++) perm[i] = (int)i;
+      std::stable_sort(perm.begin(), perm.end(), [&](int x, int y) {
+        const nsm::EnsTile &a = tile_work[(size_t)x], &b = tile_work[(size_t)y];
+        if (mesh_of(a.s) != mesh_of(b.s)) return mesh_of(a.s) < mesh_of(b.s);
+        return a.t < b.t;
+      });
+      std::vector<nsm::EnsTile> reordered(tile_work.size());
+      std::vector<int>          pos(tile_work.size());
+      for (size_t i = 0; i < perm.size(); i++) {
+        reordered[i]         = tile_work[(size_t)perm[i]];
+        pos[(size_t)perm[i]] = (int)i;
+      }
+      tile_work.swap(reordered);
+      for (int& k : fold_idx)
+        if (mixed && k < (int)tile_work.size()) k = pos[(size_t)k];
+    }
+  }
   es->rx_stride = mixed ? (int64_t)tile_work.size() + (int64_t)bnd_groups.size() : (int64_t)node_chunks.size();
 
   // ---- allocations and uploads
