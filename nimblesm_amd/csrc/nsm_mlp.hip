@@ -62,16 +62,22 @@ __device__ __forceinline__ float gelu_grad(float x)
 // ---------------------------------------------------------------------------
 // GEMM: C[m][n] = sum_k A(m, k) B(k, n), A(m, k) = a[m a_m + k a_k],
 // B(k, n) = b[k b_k + n b_n] (BF16), FP32 accumulation, with an epilogue.
-// A 256-thread workgroup computes a 64 x 64 tile of C: four waves of 32 x 32,
-// each 2 x 2 MFMA tiles of 16 x 16, over K in steps of 32 staged in LDS.
+// The network's products are small (M <= a few thousand rows, N, K <= 512),
+// so a product's time is the latency of its loads, not its multiplies: a
+// 256-thread workgroup computes a 32 x 32 tile of C (four waves of one
+// 16 x 16 MFMA tile each) and stages up to 256 of K at once, every load of
+// that range in flight together, so a product with K <= 256 waits for one
+// round trip to memory instead of one per 32 of K.
 // MFMA 16x16x32 operand layout: lane l holds row (A) / column (B) l % 16 and
 // k = 8 (l / 16) .. 8 (l / 16) + 7; the result holds column l % 16, rows
 // 4 (l / 16) .. 4 (l / 16) + 3.
 // ---------------------------------------------------------------------------
-constexpr int kTile = 64, kKStep = 32, kPad = 8, kGemmThreads = 256;
-// weight-gradient products (K = batch) are split into at most this many
-// K ranges so that more workgroups share the long reduction
-constexpr int kMaxSplits = 8;
+constexpr int kTile = 32, kKStep = 32, kKStage = 256, kPad = 8, kGemmThreads = 256;
+// chunks of 8 values one thread loads per operand and K stage
+constexpr int kChunks = kTile * kKStage / 8 / kGemmThreads;
+// weight-gradient products (K = batch) longer than this are split into at
+// most kMaxSplits K ranges summed by k_split_sum
+constexpr int kSplitAbove = 1024, kMaxSplits = 8;
 
 enum Epilogue {
   kStoreC     = 0,  // C = v (FP32)
@@ -80,9 +86,9 @@ enum Epilogue {
   kBackHidden = 3,  // g = v * gelu'(Z): G = g (FP32), D = bf16(g)
 };
 
-// How a 64 x 32 operand tile is staged: 16-byte loads along k (contiguous
-// in k), 16-byte loads along m / n written transposed into LDS (contiguous
-// in m / n), or one element at a time.
+// How an operand tile is staged: 16-byte loads along k (contiguous in k),
+// 16-byte loads along m / n written transposed into LDS (contiguous in m /
+// n), or one element at a time.
 enum Staging { kAlongK = 1, kAlongMN = 2, kScalar = 0 };
 
 struct GemmArgs {
@@ -101,122 +107,177 @@ struct GemmArgs {
   int64_t      split_stride;
 };
 
-// Staging of one operand tile (64 rows r of the M or N side, 32 k) into
-// registers: thread t holds 8 values.  AlongK: row t / 4, k = 8 (t % 4) + j;
-// AlongMN: k = t / 8, rows 8 (t % 8) + j.
+// One operand's share of a K stage: the 32 rows r0.. of the M or N side and
+// k in [k0, k0 + 32 nk32) are cut into chunks of 8 values; chunk c of
+// thread t is t + 256 c.  AlongK / scalar: chunk = (row, 8 k); AlongMN:
+// chunk = (k, 8 rows).  Values past R or K are zero.
 struct StageRegs {
-  bf16x8 v;
+  bf16x8 v[kChunks];
 };
 
-__device__ __forceinline__ StageRegs stage_load(const bf16* p, int64_t s_r, int64_t s_k, int mode, int r0, int R,
-                                                int k0, int K)
+__device__ __forceinline__ void chunk_coords(int mode, int q, int& r, int& k)
 {
-  const int t = (int)threadIdx.x;
-  StageRegs g;
-  if (mode == kAlongK) {
-    const int r = r0 + (t >> 2), k = k0 + (t & 3) * 8;
-    if (r < R && k + 8 <= K) {
-      g.v = *reinterpret_cast<const bf16x8*>(p + (int64_t)r * s_r + k);
-      return g;
-    }
-#pragma unroll
-    for (int j = 0; j < 8; j++) g.v[j] = (r < R && k + j < K) ? p[(int64_t)r * s_r + (k + j)] : (bf16)0.0f;
-  } else if (mode == kAlongMN) {
-    const int k = k0 + (t >> 3), r = r0 + (t & 7) * 8;
-    if (k < K && r + 8 <= R) {
-      g.v = *reinterpret_cast<const bf16x8*>(p + (int64_t)k * s_k + r);
-      return g;
-    }
-#pragma unroll
-    for (int j = 0; j < 8; j++) g.v[j] = (k < K && r + j < R) ? p[(int64_t)k * s_k + (r + j)] : (bf16)0.0f;
+  if (mode == kAlongMN) {
+    k = q >> 2;
+    r = (q & 3) * 8;
   } else {
-    const int r = r0 + (t >> 2), k = k0 + (t & 3) * 8;
-#pragma unroll
-    for (int j = 0; j < 8; j++)
-      g.v[j] = (r < R && k + j < K) ? p[(int64_t)r * s_r + (int64_t)(k + j) * s_k] : (bf16)0.0f;
+    r = q & (kTile - 1);
+    k = (q >> 5) * 8;
   }
-  return g;
+}
+
+__device__ __forceinline__ void stage_load(const bf16* p, int64_t s_r, int64_t s_k, int mode, int r0, int R, int k0,
+                                           int K, int nchunks, StageRegs& g)
+{
+#pragma unroll
+  for (int c = 0; c < kChunks; c++) {
+    const int q = (int)threadIdx.x + c * kGemmThreads;
+    if (q >= nchunks) continue;
+    int rl, kl;
+    chunk_coords(mode, q, rl, kl);
+    const int r = r0 + rl, k = k0 + kl;
+    if (mode == kAlongK) {
+      if (r < R && k + 8 <= K) {
+        g.v[c] = *reinterpret_cast<const bf16x8*>(p + (int64_t)r * s_r + k);
+        continue;
+      }
+#pragma unroll
+      for (int j = 0; j < 8; j++) g.v[c][j] = (r < R && k + j < K) ? p[(int64_t)r * s_r + (k + j)] : (bf16)0.0f;
+    } else if (mode == kAlongMN) {
+      if (k < K && r + 8 <= R) {
+        g.v[c] = *reinterpret_cast<const bf16x8*>(p + (int64_t)k * s_k + r);
+        continue;
+      }
+#pragma unroll
+      for (int j = 0; j < 8; j++) g.v[c][j] = (k < K && r + j < R) ? p[(int64_t)k * s_k + (r + j)] : (bf16)0.0f;
+    } else {
+#pragma unroll
+      for (int j = 0; j < 8; j++)
+        g.v[c][j] = (r < R && k + j < K) ? p[(int64_t)r * s_r + (int64_t)(k + j) * s_k] : (bf16)0.0f;
+    }
+  }
 }
 
 template <int kLd>
-__device__ __forceinline__ void stage_store(const StageRegs& g, int mode, bf16 (*S)[kLd])
+__device__ __forceinline__ void stage_store(const StageRegs& g, int mode, int nchunks, bf16 (*S)[kLd])
 {
-  const int t = (int)threadIdx.x;
-  if (mode == kAlongMN) {
-    const int k = t >> 3, r = (t & 7) * 8;
 #pragma unroll
-    for (int j = 0; j < 8; j++) S[r + j][k] = g.v[j];
-  } else {
-    *reinterpret_cast<bf16x8*>(&S[t >> 2][(t & 3) * 8]) = g.v;
+  for (int c = 0; c < kChunks; c++) {
+    const int q = (int)threadIdx.x + c * kGemmThreads;
+    if (q >= nchunks) continue;
+    int r, k;
+    chunk_coords(mode, q, r, k);
+    if (mode == kAlongMN) {
+#pragma unroll
+      for (int j = 0; j < 8; j++) S[r + j][k] = g.v[c][j];
+    } else {
+      *reinterpret_cast<bf16x8*>(&S[r][k]) = g.v[c];
+    }
+  }
+}
+
+// Tile (bx, by) of C (columns 32 bx.., rows 32 by..) over split bz of the
+// K range, by the whole workgroup (bx, by, bz uniform).
+__device__ __forceinline__ void gemm_tile(const GemmArgs& g, int mode, int bx, int by, int bz)
+{
+  constexpr int kLd = kKStage + kPad;
+  __shared__ __attribute__((aligned(16))) bf16 As[kTile][kLd];
+  __shared__ __attribute__((aligned(16))) bf16 Bs[kTile][kLd];
+  const int lane = (int)threadIdx.x & 63, wave = (int)threadIdx.x >> 6;
+  const int m0 = by * kTile, n0 = bx * kTile;
+  const int wm = wave >> 1, wn = wave & 1;
+  const int kb = bz * g.k_chunk, ke = min(g.K, kb + g.k_chunk);
+  f32x4     acc = f32x4{0.f, 0.f, 0.f, 0.f};
+
+  for (int k0 = kb; k0 < ke; k0 += kKStage) {
+    const int nk32    = (min(kKStage, ke - k0) + kKStep - 1) / kKStep;  // 32-wide steps of this stage
+    const int nchunks = kTile * nk32 * kKStep / 8;
+    StageRegs ra, rb;
+    stage_load(g.a, g.a_m, g.a_k, g.a_stage, m0, g.M, k0, ke, nchunks, ra);
+    stage_load(g.b, g.b_n, g.b_k, g.b_stage, n0, g.N, k0, ke, nchunks, rb);
+    if (k0 > kb) __syncthreads();  // the previous stage's LDS reads are done
+    stage_store<kLd>(ra, g.a_stage, nchunks, As);
+    stage_store<kLd>(rb, g.b_stage, nchunks, Bs);
+    __syncthreads();
+    for (int s = 0; s < nk32; s++) {
+      const bf16x8 af  = *reinterpret_cast<const bf16x8*>(&As[wm * 16 + (lane & 15)][s * kKStep + 8 * (lane >> 4)]);
+      const bf16x8 bfr = *reinterpret_cast<const bf16x8*>(&Bs[wn * 16 + (lane & 15)][s * kKStep + 8 * (lane >> 4)]);
+      acc              = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af, bfr, acc, 0, 0, 0);
+    }
+  }
+
+  float* const c = g.c + (int64_t)bz * g.split_stride;
+  const int    n = n0 + wn * 16 + (lane & 15);
+#pragma unroll
+  for (int r = 0; r < 4; r++) {
+    const int m = m0 + wm * 16 + 4 * (lane >> 4) + r;
+    if (m >= g.M || n >= g.N) continue;
+    const int64_t o = (int64_t)m * g.ldc + n;
+    float         v = acc[r];
+    if (mode == kStoreC) {
+      c[o] = v;
+    } else if (mode == kHidden) {
+      v += g.bias[n];
+      c[o]   = v;
+      g.h[o] = (bf16)gelu(v);
+    } else if (mode == kOutput) {
+      c[o] = v + g.bias[n];
+    } else {  // kBackHidden
+      const float gr = v * gelu_grad(g.z[o]);
+      c[o]           = gr;
+      g.h[o]         = (bf16)gr;
+    }
   }
 }
 
 __global__ __launch_bounds__(kGemmThreads) void k_gemm_bf16(GemmArgs g, int mode)
 {
-  constexpr int kLd = kKStep + kPad;
-  __shared__ __attribute__((aligned(16))) bf16 As[kTile][kLd];
-  __shared__ __attribute__((aligned(16))) bf16 Bs[kTile][kLd];
-  const int lane = (int)threadIdx.x & 63, wave = (int)threadIdx.x >> 6;
-  const int m0 = (int)blockIdx.y * kTile, n0 = (int)blockIdx.x * kTile;
-  const int wm = wave >> 1, wn = wave & 1;
-  // split z of the K range
-  const int kb = (int)blockIdx.z * g.k_chunk, ke = min(g.K, kb + g.k_chunk);
-  f32x4     acc[2][2];
-#pragma unroll
-  for (int i = 0; i < 2; i++)
-#pragma unroll
-    for (int j = 0; j < 2; j++) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+  gemm_tile(g, mode, (int)blockIdx.x, (int)blockIdx.y, (int)blockIdx.z);
+}
 
-  // the next K step's operands are loaded while the current one multiplies
-  StageRegs ra = stage_load(g.a, g.a_m, g.a_k, g.a_stage, m0, g.M, kb, ke);
-  StageRegs rb = stage_load(g.b, g.b_n, g.b_k, g.b_stage, n0, g.N, kb, ke);
-  for (int k0 = kb; k0 < ke; k0 += kKStep) {
-    stage_store<kLd>(ra, g.a_stage, As);
-    stage_store<kLd>(rb, g.b_stage, Bs);
-    __syncthreads();
-    if (k0 + kKStep < ke) {
-      ra = stage_load(g.a, g.a_m, g.a_k, g.a_stage, m0, g.M, k0 + kKStep, ke);
-      rb = stage_load(g.b, g.b_n, g.b_k, g.b_stage, n0, g.N, k0 + kKStep, ke);
+// db[n] = sum over the batch of G[b][n] for the four columns 4 w .. 4 w + 3
+// of workgroup w: one wave per column, lane l sums rows l, l + 64, ... in
+// ascending order, then a fixed shuffle tree
+__device__ __forceinline__ void bias_grad_columns(const float* __restrict__ G, int64_t B, int N,
+                                                  float* __restrict__ db, int w)
+{
+  const int n = w * 4 + (int)(threadIdx.x >> 6);
+  if (n >= N) return;
+  const int lane = (int)threadIdx.x & 63;
+  float     s    = 0.0f;
+  for (int64_t b = lane; b < B; b += 64) s += G[b * N + n];
+#pragma unroll
+  for (int m = 32; m > 0; m >>= 1) s += __shfl_down(s, m, 64);
+  if (lane == 0) db[n] = s;
+}
+
+// The independent parts of one layer's backward pass in one launch:
+// workgroups [0, tiles[0]) compute tiles of product 0 (the weight
+// gradient), the next tiles[1] tiles of product 1 (the input gradient,
+// absent below the first layer), the rest the bias gradient.  Products are
+// unsplit (K = batch <= kSplitAbove).
+struct LayerBackward {
+  GemmArgs     prod[2];
+  int          mode[2];
+  int          tiles[2], tiles_x[2];
+  const float* G;  // [B][N] FP32 gradient of the layer's output
+  int64_t      B;
+  int          N;
+  float*       db;
+};
+
+__global__ __launch_bounds__(kGemmThreads) void k_layer_backward(LayerBackward lb)
+{
+  int w = (int)blockIdx.x;
+#pragma unroll
+  for (int q = 0; q < 2; q++) {
+    if (w < lb.tiles[q]) {  // uniform across the workgroup
+      gemm_tile(lb.prod[q], lb.mode[q], w % lb.tiles_x[q], w / lb.tiles_x[q], 0);
+      return;
     }
-    bf16x8 af[2], bfr[2];
-#pragma unroll
-    for (int i = 0; i < 2; i++) {
-      af[i]  = *reinterpret_cast<const bf16x8*>(&As[wm * 32 + i * 16 + (lane & 15)][8 * (lane >> 4)]);
-      bfr[i] = *reinterpret_cast<const bf16x8*>(&Bs[wn * 32 + i * 16 + (lane & 15)][8 * (lane >> 4)]);
-    }
-#pragma unroll
-    for (int i = 0; i < 2; i++)
-#pragma unroll
-      for (int j = 0; j < 2; j++) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[i], bfr[j], acc[i][j], 0, 0, 0);
-    __syncthreads();
+    w -= lb.tiles[q];
   }
-
-  float* const c = g.c + (int64_t)blockIdx.z * g.split_stride;
-#pragma unroll
-  for (int i = 0; i < 2; i++)
-#pragma unroll
-    for (int j = 0; j < 2; j++)
-#pragma unroll
-      for (int r = 0; r < 4; r++) {
-        const int m = m0 + wm * 32 + i * 16 + 4 * (lane >> 4) + r;
-        const int n = n0 + wn * 32 + j * 16 + (lane & 15);
-        if (m >= g.M || n >= g.N) continue;
-        const int64_t o = (int64_t)m * g.ldc + n;
-        float         v = acc[i][j][r];
-        if (mode == kStoreC) {
-          c[o] = v;
-        } else if (mode == kHidden) {
-          v += g.bias[n];
-          c[o]   = v;
-          g.h[o] = (bf16)gelu(v);
-        } else if (mode == kOutput) {
-          c[o] = v + g.bias[n];
-        } else {  // kBackHidden
-          const float gr = v * gelu_grad(g.z[o]);
-          c[o]           = gr;
-          g.h[o]         = (bf16)gr;
-        }
-      }
+  bias_grad_columns(lb.G, lb.B, lb.N, lb.db, w);
 }
 
 // out[i] = sum over splits s of part[s * stride + i], ascending s
@@ -263,29 +324,23 @@ __global__ __launch_bounds__(256) void k_mse(const float* __restrict__ y, const 
   if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
 }
 
-// loss = sum of the kMseBlocks partials (ascending) / n
+// loss of pass p = sum of its kMseBlocks partials (ascending) / n, for
+// passes 0 .. gridDim.x - 1 (partial[p kMseBlocks + b])
 __global__ __launch_bounds__(64) void k_mse_total(const double* __restrict__ partial, int64_t n,
                                                   double* __restrict__ loss_out)
 {
   if (threadIdx.x != 0) return;
-  double s = 0.0;
-  for (int b = 0; b < kMseBlocks; b++) s += partial[b];
-  *loss_out = s / (double)n;
+  const double* p = partial + (int64_t)blockIdx.x * kMseBlocks;
+  double        s = 0.0;
+  for (int b = 0; b < kMseBlocks; b++) s += p[b];
+  loss_out[blockIdx.x] = s / (double)n;
 }
 
-// db[n] = sum over the batch of G[b][n]: one wave per column, lane l sums
-// rows l, l + 64, ... in ascending order, then a fixed shuffle tree
+// bias gradient alone (batches longer than kSplitAbove)
 __global__ __launch_bounds__(256) void k_bias_grad(const float* __restrict__ G, int64_t B, int N,
                                                    float* __restrict__ db)
 {
-  const int n = (int)(blockIdx.x * 4 + (threadIdx.x >> 6));
-  if (n >= N) return;
-  const int lane = (int)threadIdx.x & 63;
-  float     s    = 0.0f;
-  for (int64_t b = lane; b < B; b += 64) s += G[b * N + n];
-#pragma unroll
-  for (int w = 32; w > 0; w >>= 1) s += __shfl_down(s, w, 64);
-  if (lane == 0) db[n] = s;
+  bias_grad_columns(G, B, N, db, (int)blockIdx.x);
 }
 
 // Adam on the FP32 master parameters; BF16 copies for the forward pass.
@@ -336,7 +391,7 @@ struct nsm_mlp {
   float*               t32 = nullptr;  // targets [cap][d_L]
   double*              loss = nullptr; // [loss_cap]
   int64_t              loss_cap = 0;
-  double*              loss_part = nullptr;  // [kMseBlocks]
+  double*              loss_part = nullptr;  // [min(loss_cap, kLossChunk)][kMseBlocks]
   float*               split = nullptr;      // split-K partial products
   int64_t              split_cap = 0;
 
@@ -355,15 +410,23 @@ int staging(const bf16* p, int64_t s_r, int64_t s_k)
   return kScalar;
 }
 
-// C = A B with the epilogue `mode`.  splits > 1 (kStoreC only): the K range
-// is cut into `splits` parts written to net->split (one M x N block each)
-// and summed in ascending order into a.c.
-int mlp_gemm(nsm_mlp* net, GemmArgs a, int mode, int splits = 1)
+// Staging modes and K ranges of a product cut into `splits` K ranges
+// (clamped to its 32-wide steps); returns the split count.
+int prepare(GemmArgs& a, int splits)
 {
   a.a_stage = staging(a.a, a.a_m, a.a_k);
   a.b_stage = staging(a.b, a.b_n, a.b_k);
   splits    = std::max(1, std::min(splits, (a.K + kKStep - 1) / kKStep));
   a.k_chunk = ((a.K + splits - 1) / splits + kKStep - 1) / kKStep * kKStep;
+  return splits;
+}
+
+// C = A B with the epilogue `mode`.  splits > 1 (kStoreC only): the K range
+// is cut into `splits` parts written to net->split (one M x N block each)
+// and summed in ascending order into a.c.
+int mlp_gemm(nsm_mlp* net, GemmArgs a, int mode, int splits = 1)
+{
+  splits         = prepare(a, splits);
   float* final_c = a.c;
   const int64_t mn = (int64_t)a.M * a.ldc;
   if (splits > 1) {
@@ -455,19 +518,19 @@ int forward(nsm_mlp* net, int64_t B, bool convert_input = true)
   return NSM_OK;
 }
 
-// Loss of the forward pass against t32 and the gradients of every parameter
-// into net->g; loss written to loss_out (device) when given.
-int backward(nsm_mlp* net, int64_t B, double* loss_out)
+// Gradients of every parameter into net->g from the forward pass against
+// t32; the pass's kMseBlocks squared-error partials go to `partial`
+// (device; k_mse_total turns them into the loss).  With at most
+// kSplitAbove batch rows each layer is one launch (k_layer_backward: weight
+// gradient, input gradient and bias gradient side by side), else one per
+// product.
+int backward(nsm_mlp* net, int64_t B, double* partial)
 {
   const int L  = net->L();
   const int dL = net->width[(size_t)L];
   hipLaunchKernelGGL(k_mse, dim3(kMseBlocks), dim3(256), 0, net->stream, net->z[(size_t)L - 1], net->t32, B * dL,
-                     net->gf, net->gb, net->loss_part);
+                     net->gf, net->gb, partial);
   MLP_LAUNCHED();
-  if (loss_out) {
-    hipLaunchKernelGGL(k_mse_total, dim3(1), dim3(64), 0, net->stream, net->loss_part, B * dL, loss_out);
-    MLP_LAUNCHED();
-  }
   float* gcur = net->gf;
   bf16*  dcur = net->gb;
   float* gnext = net->gf2;
@@ -487,13 +550,6 @@ int backward(nsm_mlp* net, int64_t B, double* loss_out)
     a.b_n = 1;
     a.c   = net->g + net->w_off[(size_t)l - 1];
     a.ldc = din;
-    // about 64 batch rows per split
-    int st = mlp_gemm(net, a, kStoreC, (int)std::min<int64_t>(kMaxSplits, std::max<int64_t>(1, B / 64)));
-    if (st) return st;
-    hipLaunchKernelGGL(k_bias_grad, dim3(blocks_of(dout, 4)), dim3(256), 0, net->stream, gcur, B, dout,
-                       net->g + net->b_off[(size_t)l - 1]);
-    MLP_LAUNCHED();
-    if (l == 1) break;
     // G_{l-1}[b][k] = (sum_n D_l[b][n] W_l[n][k]) gelu'(Z_{l-1}[b][k])
     GemmArgs d{};
     d.M   = (int)B;
@@ -508,8 +564,37 @@ int backward(nsm_mlp* net, int64_t B, double* loss_out)
     d.c   = gnext;
     d.ldc = din;
     d.h   = dnext;
-    d.z   = net->z[(size_t)l - 2];
-    if ((st = mlp_gemm(net, d, kBackHidden))) return st;
+    d.z   = l > 1 ? net->z[(size_t)l - 2] : nullptr;
+    float* const db = net->g + net->b_off[(size_t)l - 1];
+    if (B <= kSplitAbove) {
+      LayerBackward lb{};
+      lb.prod[0]    = a;
+      lb.mode[0]    = kStoreC;
+      prepare(lb.prod[0], 1);
+      lb.tiles_x[0] = (int)blocks_of(a.N, kTile);
+      lb.tiles[0]   = lb.tiles_x[0] * (int)blocks_of(a.M, kTile);
+      if (l > 1) {
+        lb.prod[1]    = d;
+        lb.mode[1]    = kBackHidden;
+        prepare(lb.prod[1], 1);
+        lb.tiles_x[1] = (int)blocks_of(d.N, kTile);
+        lb.tiles[1]   = lb.tiles_x[1] * (int)blocks_of(d.M, kTile);
+      }
+      lb.G  = gcur;
+      lb.B  = B;
+      lb.N  = dout;
+      lb.db = db;
+      const unsigned grid = (unsigned)(lb.tiles[0] + lb.tiles[1]) + blocks_of(dout, 4);
+      hipLaunchKernelGGL(k_layer_backward, dim3(grid), dim3(kGemmThreads), 0, net->stream, lb);
+      MLP_LAUNCHED();
+    } else {
+      const int splits = (int)std::min<int64_t>(kMaxSplits, (B + kSplitAbove - 1) / kSplitAbove);
+      int       st     = mlp_gemm(net, a, kStoreC, splits);
+      if (st) return st;
+      hipLaunchKernelGGL(k_bias_grad, dim3(blocks_of(dout, 4)), dim3(256), 0, net->stream, gcur, B, dout, db);
+      MLP_LAUNCHED();
+      if (l > 1 && (st = mlp_gemm(net, d, kBackHidden))) return st;
+    }
     std::swap(gcur, gnext);
     std::swap(dcur, dnext);
   }
@@ -550,6 +635,27 @@ int load_rows(nsm_mlp* net, int64_t B, const float* x, const float* t)
   return NSM_OK;
 }
 
+// squared-error partials of up to kLossChunk passes are kept on the device
+// and turned into losses by one k_mse_total launch per chunk
+constexpr int kLossChunk = 4096;
+
+// losses for `passes` forward/backward passes and partials for
+// min(passes, kLossChunk) of them
+int ensure_loss(nsm_mlp* net, int64_t passes)
+{
+  if (passes <= net->loss_cap) return NSM_OK;
+  if (net->loss) MLP_HIP(hipFree(net->loss));
+  if (net->loss_part) MLP_HIP(hipFree(net->loss_part));
+  net->loss      = nullptr;
+  net->loss_part = nullptr;
+  net->loss_cap  = 0;
+  MLP_HIP(hipMalloc((void**)&net->loss, (size_t)passes * sizeof(double)));
+  MLP_HIP(hipMalloc((void**)&net->loss_part,
+                    (size_t)(std::min<int64_t>(passes, kLossChunk) * kMseBlocks) * sizeof(double)));
+  net->loss_cap = passes;
+  return NSM_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -585,8 +691,7 @@ int nsm_mlp_create(int num_layers, const int* widths, uint64_t seed, nsm_mlp** o
   if (hipMalloc((void**)&net->p, n * 4) != hipSuccess || hipMalloc((void**)&net->pb, n * 2) != hipSuccess ||
       hipMalloc((void**)&net->g, n * 4) != hipSuccess || hipMalloc((void**)&net->m1, n * 4) != hipSuccess ||
       hipMalloc((void**)&net->m2, n * 4) != hipSuccess ||
-      hipMalloc((void**)&net->split, (size_t)net->split_cap * 4) != hipSuccess ||
-      hipMalloc((void**)&net->loss_part, kMseBlocks * sizeof(double)) != hipSuccess)
+      hipMalloc((void**)&net->split, (size_t)net->split_cap * 4) != hipSuccess)
     return cleanup(fail(NSM_ERR_NOMEM, "device allocation of the parameters failed"));
   // uniform(-a, a) weights with a = sqrt(6 / (fan_in + fan_out)), zero biases
   std::vector<float> init(n, 0.0f);
@@ -662,11 +767,11 @@ int nsm_mlp_loss_gradients(nsm_mlp* net, int64_t batch, const float* x, const fl
   MLP_HIP(hipSetDevice(net->device));
   int st = load_rows(net, batch, x, t);
   if (st) return st;
-  if (!net->loss) {
-    MLP_HIP(hipMalloc((void**)&net->loss, sizeof(double)));
-    net->loss_cap = 1;
-  }
-  if ((st = forward(net, batch)) || (st = backward(net, batch, net->loss))) return st;
+  const int64_t n = batch * net->width.back();
+  if ((st = ensure_loss(net, 1)) || (st = forward(net, batch)) || (st = backward(net, batch, net->loss_part)))
+    return st;
+  hipLaunchKernelGGL(k_mse_total, dim3(1), dim3(64), 0, net->stream, net->loss_part, n, net->loss);
+  MLP_LAUNCHED();
   MLP_HIP(hipMemcpyAsync(loss, net->loss, sizeof(double), hipMemcpyDeviceToHost, net->stream));
   MLP_HIP(hipMemcpyAsync(grad, net->g, (size_t)net->num_params * sizeof(float), hipMemcpyDeviceToHost,
                          net->stream));
@@ -683,12 +788,7 @@ int nsm_mlp_train(nsm_mlp* net, int64_t batch, const float* x, const float* t, i
   MLP_HIP(hipSetDevice(net->device));
   int st = load_rows(net, batch, x, t);
   if (st) return st;
-  if (iterations > net->loss_cap) {
-    if (net->loss) MLP_HIP(hipFree(net->loss));
-    net->loss = nullptr;
-    MLP_HIP(hipMalloc((void**)&net->loss, (size_t)iterations * sizeof(double)));
-    net->loss_cap = iterations;
-  }
+  if ((st = ensure_loss(net, iterations))) return st;
   const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
   for (int it = 0; it < iterations; it++) {
     // cosine decay from lr to lr_final
@@ -697,10 +797,19 @@ int nsm_mlp_train(nsm_mlp* net, int64_t batch, const float* x, const float* t, i
     net->adam_t++;
     const float c1 = (float)(1.0 - std::pow((double)b1, (double)net->adam_t));
     const float c2 = (float)(1.0 - std::pow((double)b2, (double)net->adam_t));
-    if ((st = forward(net, batch, it == 0)) || (st = backward(net, batch, net->loss + it))) return st;
+    if ((st = forward(net, batch, it == 0)) ||
+        (st = backward(net, batch, net->loss_part + (int64_t)(it % kLossChunk) * kMseBlocks)))
+      return st;
     hipLaunchKernelGGL(k_adam, dim3(blocks_of(net->num_params, 256)), dim3(256), 0, net->stream, net->p, net->g,
                        net->m1, net->m2, net->pb, net->num_params, lri, b1, b2, c1, c2, eps);
     MLP_LAUNCHED();
+    // the losses of a full chunk of iterations (or of the last one)
+    if ((it + 1) % kLossChunk == 0 || it + 1 == iterations) {
+      const int first = it - it % kLossChunk;
+      hipLaunchKernelGGL(k_mse_total, dim3((unsigned)(it + 1 - first)), dim3(64), 0, net->stream, net->loss_part,
+                         batch * net->width.back(), net->loss + first);
+      MLP_LAUNCHED();
+    }
   }
   if (loss_history && iterations > 0)
     MLP_HIP(hipMemcpyAsync(loss_history, net->loss, (size_t)iterations * sizeof(double), hipMemcpyDeviceToHost,

@@ -51,7 +51,8 @@ def test_initial_parameters_match_the_oracle(hip_device):
     net.close()
 
 
-@pytest.mark.parametrize("widths,B", [((37, 70), 100), ((4, 256), 512), ((4, 48, 40), 96), ((16, 96, 80), 200)])
+@pytest.mark.parametrize("widths,B", [((37, 70), 100), ((4, 256), 512), ((4, 48, 40), 96), ((16, 96, 80), 200),
+                                      ((37, 70), 2500)])  # weight gradients over 3 K ranges
 def test_gradients_small_networks(hip_device, widths, B):
     net = MLP(widths, seed=7)
     x, t = data(widths, B, 1)
@@ -116,3 +117,23 @@ def test_training_fits_a_smooth_map(hip_device):
     y = net.forward(x)
     assert np.linalg.norm(y - t) / np.linalg.norm(t) < 0.05
     net.close()
+
+
+def test_loss_history_across_calls_and_chunks(hip_device):
+    """Losses are formed from per-iteration partials in chunks of 4096
+    iterations: at a constant learning rate one call of 4100 iterations has
+    the same loss history, bit for bit, as calls of 4096 and 4 (Adam's step
+    count carries over between calls), and its first loss is the one
+    loss_gradients reports."""
+    widths = (4, 24, 8)
+    x, t = data(widths, 40, 6)
+    a, b = MLP(widths, seed=2), MLP(widths, seed=2)
+    loss0, _ = b.loss_gradients(x, t)
+    h = a.train(x, t, 4100, 1e-3, 1e-3)
+    h1 = b.train(x, t, 4096, 1e-3, 1e-3)
+    h2 = b.train(x, t, 4, 1e-3, 1e-3)
+    assert h[0] == loss0
+    assert np.array_equal(h, np.concatenate([h1, h2]))
+    assert np.array_equal(a.get_parameters(), b.get_parameters())
+    a.close()
+    b.close()
