@@ -661,7 +661,6 @@ int build_device(nsm_ensemble* es)
     ta.num_bnd         = 0;
     ta.bnd_base        = 0;
     float* partials = nullptr;
-    float* sink     = nullptr;
     if ((r = upload(B, elem_off, (int32_t**)&ta.elem_off, &tot, st)) ||
         (r = upload(B, node_off, (int32_t**)&ta.node_off, &tot, st)) ||
         (r = upload(B, nodes, (int32_t**)&ta.nodes, &tot, st)) ||
@@ -674,11 +673,9 @@ int build_device(nsm_ensemble* es)
         (r = upload(B, tile_block, &es->d_tile_block, &tot, st)) ||
         (r = upload(B, tile_work, &es->d_tiles, &tot, st)) ||
         (r = upload(B, bnd_groups, &es->d_bnd_groups, &tot, st)) ||
-        (r = dalloc(B, &partials, 3 * (size_t)std::max<int64_t>(partial_base, 1), &tot)) ||
-        (r = dalloc(B, &sink, (size_t)8 * 120 * es->tile_elems, &tot)))
+        (r = dalloc(B, &partials, 3 * (size_t)std::max<int64_t>(partial_base, 1), &tot)))
       return r;
     ta.partials = partials;
-    ta.ip_sink  = sink;
   }
   ENS_HIP(hipStreamSynchronize(st));
   if ((r = upload_parameters(es))) return r;

@@ -687,10 +687,6 @@ int setup_tiles(nsm_model* md)
   if ((st = dalloc(&partials, std::max<size_t>(3 * ts.bnd_ent.size(), 1), &md->device_bytes))) return st;
   md->tile_bufs.push_back((void*)partials);
   ta.partials = partials;
-  float* sink = nullptr;
-  if ((st = dalloc(&sink, (size_t)8 * 120 * ts.tile_elems, &md->device_bytes))) return st;
-  md->tile_bufs.push_back((void*)sink);
-  ta.ip_sink = sink;
   md->blk_tile_off.assign((size_t)md->num_blocks + 1, 0);
   for (int b = 0; b <= md->num_blocks; b++)
     md->blk_tile_off[(size_t)b] =

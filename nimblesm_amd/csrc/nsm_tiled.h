@@ -29,7 +29,6 @@ struct TileArgs {
   int             bnd_base;    // device id of boundary node 0 (boundary node b is bnd_base + b)
   const int32_t*  bnd_ptr;     // [B+1]
   float*          partials;    // [partials][3]; node b's are [bnd_ptr[b], bnd_ptr[b+1])
-  float*          ip_sink;     // [8][120][tile_elems]: F / sigma stores of launches that keep none
 };
 
 // What one tile of an element launch works on: tile t of the tile tables,
@@ -80,14 +79,14 @@ unsigned      boundary_grid(int num_bnd);
 // Workgroup 0 first writes the folded reaction total of a previous step.
 // F and sigma are tile-blocked: component c, point q of local element l of
 // tile t (elements [e0, e0 + n)) at Fip[72 e0 + (4 c + q/2) 2n + 2l + q%2]
-// and sip[48 e0 + (4 c + q/2) 2n + 2l + q%2]; with Fip = sip = nullptr they
-// go to ta.ip_sink (overwritten, cache resident) instead.
+// and sip[48 e0 + (4 c + q/2) 2n + 2l + q%2]; with Fip = sip = nullptr the
+// launch keeps neither.
 void launch_element_tiled(const BlockTable<float>& bt, const TileArgs& ta, int t_begin, int t_count,
                           const double* X, const NodeArrays& na, float* Fip, float* sip, const StepArgs& sa,
                           ReactionArgs rx, ReactionFold fold, int* status, hipStream_t s);
 // The same kernel over the tiles of every sample of an ensemble (ta: the
 // concatenated tile tables of its meshes; na: all samples' node rows, X and
-// flags of all meshes); F and sigma go to the sink.
+// flags of all meshes); it keeps no F or sigma.
 void launch_ens_element_tiled(const EnsTileWork& work, const TileArgs& ta, const double* X, const NodeArrays& na,
                               ReactionArgs rx, int* status, hipStream_t s);
 // Boundary nodes of every sample: group g (ranges[g]: sample, first local

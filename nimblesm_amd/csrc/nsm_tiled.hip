@@ -264,11 +264,12 @@ __device__ __forceinline__ void fold_previous(const EnsTileWork& w, double*)
 // Inside the loop every vector load and every F / sigma store is executed on
 // every path (indices clamped), so at least the 120 stores of the element
 // arithmetic always separate a load from its first use and the waits the
-// compiler places never wait for those stores.  kStoreIp = false (off-nominal
-// UQ samples, which keep no element data) sends the same stores to a small
-// sink that stays in L2 instead of F and sigma: without them the compiler
-// schedules the element arithmetic with more registers than a wave pair
-// has, and the kernel is no faster for the bytes it saves.  The launch bound
+// compiler places never wait for those stores.  kStoreIp = false (ensembles
+// and off-nominal UQ samples, which keep no element data) stores nothing,
+// but each Gauss-point pair's stress still passes through an empty asm
+// statement where its stores would be: with no such point of use the
+// compiler schedules the element arithmetic with more registers than a wave
+// pair has and spills.  The launch bound
 // holds the kernel to two waves per SIMD (at most 256 registers): left free,
 // the scheduler takes more for the gather assembly and halves occupancy.
 template <int kThreads, bool kStoreIp, typename Work>
@@ -385,13 +386,11 @@ __global__ __launch_bounds__(kThreads, 2) void k_element_tiled(const Work work, 
       // values, element l's points 2p and 2p+1 side by side, so each tile
       // writes two contiguous blocks with 8-byte stores; non-temporal stores
       // (the step never reads them back) keep node data and partials in cache
-      float* const Ft   = kStoreIp ? Fip + (int64_t)e0 * 72 : ta.ip_sink + (t & 7) * 120 * ntl;
-      float* const St   = kStoreIp ? sip + (int64_t)e0 * 48 : Ft + 72 * ntl;
+      float* const Ft   = kStoreIp ? Fip + (int64_t)e0 * 72 : nullptr;
+      float* const St   = kStoreIp ? sip + (int64_t)e0 * 48 : nullptr;
       const auto   rF   = tile_rsrc(Ft, 72 * 4 * ntl);
       const auto   rS   = tile_rsrc(St, 48 * 4 * ntl);
       const int    loff = 8 * lcl;
-      // the sink is rewritten every 8 tiles: keep it in L2
-      constexpr int kAux  = kStoreIp ? kNonTemporal : 0;
       auto store = [&](int qm, int qp, const float* Fm, const float* Sm, const float* Fp, const float* Sp) {
         const bool   m_even = qm < qp;  // qm, qp: constants after inlining
         const int    p      = (m_even ? qm : qp) / 2;
@@ -399,10 +398,17 @@ __global__ __launch_bounds__(kThreads, 2) void k_element_tiled(const Work work, 
         const float* Fo     = m_even ? Fp : Fm;
         const float* Se     = m_even ? Sm : Sp;
         const float* So     = m_even ? Sp : Sm;
+        if (kStoreIp) {
 #pragma unroll
-        for (int c = 0; c < 9; c++) store_pair<kAux>(Fe[c], Fo[c], rF, loff, 8 * (c * 4 + p) * ntl);
+          for (int c = 0; c < 9; c++) store_pair<kNonTemporal>(Fe[c], Fo[c], rF, loff, 8 * (c * 4 + p) * ntl);
 #pragma unroll
-        for (int c = 0; c < 6; c++) store_pair<kAux>(Se[c], So[c], rS, loff, 8 * (c * 4 + p) * ntl);
+          for (int c = 0; c < 6; c++) store_pair<kNonTemporal>(Se[c], So[c], rS, loff, 8 * (c * 4 + p) * ntl);
+        } else {
+          // the pair's point of use (see above); emits no instruction
+#pragma unroll
+          for (int c = 0; c < 6; c++) asm volatile("" ::"v"(Se[c]), "v"(So[c]));
+          __builtin_amdgcn_sched_barrier(0);
+        }
       };
       if (!mixed::element(Xe, ue, Kt, Gt, store, f)) bad = 1;
     }
