@@ -267,7 +267,7 @@ __device__ __forceinline__ void element_fp64(Lds& lds, int64_t e, bool valid, do
     } else {
       double b[9];
       jacobians(k, lds.nodes.x[el], lds.nodes.u[el], a, b);
-      detb = invert3x3(b, bi);
+      detb = invert3x3<false>(b, bi);
     }
     if (kMode != kElementForcesFromStress) {
       double F[9];
@@ -402,7 +402,7 @@ __global__ __launch_bounds__(kThreads) void k_seam_defgrad(int64_t E, const doub
   if (e < E) {
     double a[9], b[9], bi[9], Fq[9];
     jacobians(k, xs[el], us[el], a, b);
-    if (invert3x3(b, bi) <= 0.0) atomicOr(status, 1);
+    if (invert3x3<false>(b, bi) <= 0.0) atomicOr(status, 1);
     def_grad(a, bi, Fq);
     for (int i = 0; i < 9; i++) F[(e * 8 + k) * 9 + i] = Fq[i];
   }

@@ -15,7 +15,68 @@ namespace fp64 {
 enum { S_XX = 0, S_YY = 1, S_ZZ = 2, S_XY = 3, S_YZ = 4, S_ZX = 5 };
 enum { F_XX = 0, F_YY = 1, F_ZZ = 2, F_XY = 3, F_YZ = 4, F_ZX = 5, F_YX = 6, F_ZY = 7, F_XZ = 8 };
 
+// ---------------------------------------------------------------------------
+// Quotients over one denominator.  The device's IEEE division of n by d
+// (the compiler's sequence for gfx950) is
+//   ds = div_scale(d), ns = div_scale(n) (with flag vcc), r = rcp(ds),
+//   two Newton steps r = fma(r, fma(-ds, r, 1), r), q = ns r,
+//   q = div_fmas(fma(-ds, q, ns), r, q, vcc), result = div_fixup(q, d, n).
+// The scale steps are identities (ds = d, ns = n, vcc = 0) whenever d and n
+// are finite, nonzero and normal with exponents far from the ends of the
+// range -- here: frexp exponents in [-299, 300] -- and div_fixup settles
+// zero, infinite and NaN operands from d and n alone, whatever quotient it
+// is given.  So for such operands r depends on d only, and the quotients
+// over one denominator share it: the same operations on the same values,
+// hence the same bits as dividing one by one, with one reciprocal (a
+// quarter-rate instruction) and its refinement per denominator instead of
+// per quotient.  A wave whose operands are not all of this kind divides
+// one by one.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double refined_rcp(double d)
+{
+  double r = __builtin_amdgcn_rcp(d);
+  double t = __builtin_fma(-d, r, 1.0);
+  r        = __builtin_fma(r, t, r);
+  t        = __builtin_fma(-d, r, 1.0);
+  return __builtin_fma(r, t, r);
+}
+
+// n / d given r = refined_rcp(d), for operands that pass plain_operand
+__device__ __forceinline__ double div_shared(double n, double d, double r)
+{
+  const double q = n * r;
+  const double e = __builtin_fma(-d, q, n);
+  return __builtin_amdgcn_div_fixup(__builtin_amdgcn_div_fmas(e, r, q, false), d, n);
+}
+
+// zero, infinite, NaN, or |x| in [2^-300, 2^300) (frexp exponent of
+// zero, infinity and NaN is 0)
+__device__ __forceinline__ bool plain_operand(double x)
+{
+  return (unsigned)(__builtin_amdgcn_frexp_exp(x) + 299) < 600u;
+}
+
+// q[i] = n[i] / d for i < kN, bit for bit as kN separate divisions
+template <int kN>
+__device__ __forceinline__ void divide_by(const double (&n)[kN], double d, double (&q)[kN])
+{
+  bool plain = plain_operand(d);
+#pragma unroll
+  for (int i = 0; i < kN; i++) plain = plain && plain_operand(n[i]);
+  if (__builtin_expect(__ballot(!plain) == 0, 1)) {  // uniform across the wave
+    const double r = refined_rcp(d);
+#pragma unroll
+    for (int i = 0; i < kN; i++) q[i] = div_shared(n[i], d, r);
+  } else {
+#pragma unroll
+    for (int i = 0; i < kN; i++) q[i] = n[i] / d;
+  }
+}
+
 // Invert3x3 on a row-major 3x3 (src/nimble_utils.h:1229-1268); returns det.
+// kShared = false divides one by one: the inverse of the reference Jacobian,
+// formed while the current Jacobian is held, is cheaper in registers that way.
+template <bool kShared = true>
 __device__ __forceinline__ double invert3x3(const double m[9], double inv[9])
 {
   const double minor0 = m[4] * m[8] - m[5] * m[7];
@@ -28,15 +89,17 @@ __device__ __forceinline__ double invert3x3(const double m[9], double inv[9])
   const double minor7 = m[0] * m[5] - m[2] * m[3];
   const double minor8 = m[0] * m[4] - m[1] * m[3];
   const double det    = m[0] * minor0 - m[1] * minor1 + m[2] * minor2;
-  inv[0] = minor0 / det;
-  inv[1] = -1.0 * minor3 / det;
-  inv[2] = minor6 / det;
-  inv[3] = -1.0 * minor1 / det;
-  inv[4] = minor4 / det;
-  inv[5] = -1.0 * minor7 / det;
-  inv[6] = minor2 / det;
-  inv[7] = -1.0 * minor5 / det;
-  inv[8] = minor8 / det;
+  const double n[9]   = {minor0,        -1.0 * minor3, minor6, -1.0 * minor1, minor4,
+                         -1.0 * minor7, minor2,        -1.0 * minor5, minor8};
+  if (kShared) {
+    double q[9];
+    divide_by(n, det, q);
+#pragma unroll
+    for (int i = 0; i < 9; i++) inv[i] = q[i];
+  } else {
+#pragma unroll
+    for (int i = 0; i < 9; i++) inv[i] = n[i] / det;
+  }
   return det;
 }
 
@@ -53,15 +116,19 @@ __device__ __forceinline__ double invert_full33(const double* a, double* inv)
   const double minor7 = a[F_XX] * a[F_YZ] - a[F_XZ] * a[F_YX];
   const double minor8 = a[F_XX] * a[F_YY] - a[F_XY] * a[F_YX];
   const double det    = a[F_XX] * minor0 - a[F_XY] * minor1 + a[F_XZ] * minor2;
-  inv[F_XX] = minor0 / det;
-  inv[F_XY] = -1.0 * minor3 / det;
-  inv[F_XZ] = minor6 / det;
-  inv[F_YX] = -1.0 * minor1 / det;
-  inv[F_YY] = minor4 / det;
-  inv[F_YZ] = -1.0 * minor7 / det;
-  inv[F_ZX] = minor2 / det;
-  inv[F_ZY] = -1.0 * minor5 / det;
-  inv[F_ZZ] = minor8 / det;
+  const double n[9]   = {minor0,        -1.0 * minor3, minor6, -1.0 * minor1, minor4,
+                         -1.0 * minor7, minor2,        -1.0 * minor5, minor8};
+  double       q[9];
+  divide_by(n, det, q);
+  inv[F_XX] = q[0];
+  inv[F_XY] = q[1];
+  inv[F_XZ] = q[2];
+  inv[F_YX] = q[3];
+  inv[F_YY] = q[4];
+  inv[F_YZ] = q[5];
+  inv[F_ZX] = q[6];
+  inv[F_ZY] = q[7];
+  inv[F_ZZ] = q[8];
   return det;
 }
 
@@ -285,9 +352,8 @@ __device__ __forceinline__ void neohookean_stress(double K, double G, const doub
   const double xj = v[S_XX] * v[S_YY] * v[S_ZZ] + 2.0 * v[S_XY] * v[S_YZ] * v[S_ZX] - v[S_XX] * v[S_YZ] * v[S_YZ] -
                     v[S_YY] * v[S_ZX] * v[S_ZX] - v[S_ZZ] * v[S_XY] * v[S_XY];
 
-  const double cj       = cbrt_ref(xj);
-  const double fac      = 1.0 / (cj * cj);
-  const double pressure = 0.5 * K * (xj - 1.0 / xj);
+  const double cj  = cbrt_ref(xj);
+  const double fac = 1.0 / (cj * cj);
 
   double bxx = v[S_XX] * v[S_XX] + v[S_XY] * v[S_XY] + v[S_ZX] * v[S_ZX];
   double byy = v[S_XY] * v[S_XY] + v[S_YY] * v[S_YY] + v[S_YZ] * v[S_YZ];
@@ -308,12 +374,17 @@ __device__ __forceinline__ void neohookean_stress(double K, double G, const doub
   byy = byy - trace / 3.0;
   bzz = bzz - trace / 3.0;
 
-  sig[S_XX] = pressure + G * bxx / xj;
-  sig[S_YY] = pressure + G * byy / xj;
-  sig[S_ZZ] = pressure + G * bzz / xj;
-  sig[S_XY] = G * bxy / xj;
-  sig[S_YZ] = G * byz / xj;
-  sig[S_ZX] = G * bzx / xj;
+  // 1/xj of the pressure and the six quotients of the deviator share xj
+  const double n[7] = {1.0, G * bxx, G * byy, G * bzz, G * bxy, G * byz, G * bzx};
+  double       q[7];
+  divide_by(n, xj, q);
+  const double pressure = 0.5 * K * (xj - q[0]);
+  sig[S_XX] = pressure + q[1];
+  sig[S_YY] = pressure + q[2];
+  sig[S_ZZ] = pressure + q[3];
+  sig[S_XY] = q[4];
+  sig[S_YZ] = q[5];
+  sig[S_ZX] = q[6];
 }
 
 }  // namespace fp64

@@ -176,3 +176,39 @@ def test_split_force_evaluation_matches_fused(hip_device):
     assert ei.value.status == 4
     for m in (fused, split, mixed):
         m.close()
+
+
+@pytest.mark.parametrize("scale", [2.0 ** -110, 2.0 ** -90, 2.0 ** 90, 2.0 ** 110])
+def test_quotients_at_extreme_exponents_bit_exact(hip_device, elems, scale):
+    """The device shares one reciprocal among the quotients over one
+    denominator (the inverses' determinants, the stress's J) when every
+    operand's exponent lies in [-299, 300], and divides one by one
+    otherwise.  Scaled deformation gradients (det F and J ~ scale^3),
+    scaled elements (det a ~ scale^3) and extreme moduli put operands on both
+    sides of that range; every output must still equal the oracle's IEEE
+    divisions bit for bit.  The reference's eigen solver (absolute
+    tolerances) returns NaN for most far-from-unit F, so the scaled-F check
+    keeps the elements whose stress is finite."""
+    import torch
+
+    X, u = elems[0][:700], elems[1][:700]
+    F0 = O.deformation_gradients(X, u)
+    Fs = F0 * scale
+    ref = O.stress(166.67e9, 76.92e9, Fs)
+    keep = np.isfinite(ref).all(axis=(1, 2))
+    assert keep.sum() >= 100
+    s = torch.zeros((int(keep.sum()), 8, 6), dtype=torch.float64, device=hip_device)
+    N.NeohookeanMaterial(7800.0, 166.67e9, 76.92e9).GetStress(_dev(Fs[keep], hip_device), s)
+    torch.cuda.synchronize()
+    assert np.array_equal(s.cpu().numpy().view(np.uint64), ref[keep].view(np.uint64))
+    for K, G in [(1e300, 1e299), (1e-300, 1e-301)]:
+        s = torch.zeros((len(X), 8, 6), dtype=torch.float64, device=hip_device)
+        N.NeohookeanMaterial(7800.0, K, G).GetStress(_dev(F0, hip_device), s)
+        torch.cuda.synchronize()
+        assert np.array_equal(s.cpu().numpy().view(np.uint64), O.stress(K, G, F0).view(np.uint64)), (K, G)
+    Xs, us = X * scale, u * scale
+    so = O.stress(1.6e11, 0.8e11, F0)
+    f = torch.zeros((len(X), 8, 3), dtype=torch.float64, device=hip_device)
+    N.HexElement.ComputeNodalForces(_dev(Xs, hip_device), _dev(us, hip_device), _dev(so, hip_device), f)
+    torch.cuda.synchronize()
+    assert np.array_equal(f.cpu().numpy().view(np.uint64), O.nodal_forces(Xs, us, so).view(np.uint64))
