@@ -59,6 +59,12 @@ VALU_PEAK_TFLOPS = {"mixed": 157.3, "fp64": 78.6}
 # 1.23k per Gauss point (Jacobians, inverses, F, polar decomposition,
 # Neo-Hookean, dN/dx, force contraction) x 8
 COUNTED_FLOPS_PER_ELEMENT_STEP = 9840
+# The FP64 element kernel is bound by VALU issue, not HBM: its VALU
+# instructions per element (8 lanes per element; rocprofv3 SQ_INSTS_VALU on
+# the 128^3 cube, 1,897 per lane) and the issue peak those are measured
+# against: 1024 SIMDs, one wave64 FP64 instruction per 4 cycles, 2.4 GHz
+FP64_VALU_INSTS_PER_ELEMENT = 8 * 1897
+VALU_ISSUE_PEAK_PER_S = 1024 * 2.4e9 / 4
 
 
 MAX_BLOCKS_PER_LAUNCH = 8  # element blocks sharing one element-kernel launch
@@ -212,6 +218,10 @@ def fp64_leg(N, workload, steps, warmup):
         "element_alg_bytes_per_launch": float(np.mean(per_launch)),
         "element_frac": float(np.mean(per_launch)) / (el_ms * 1e-3) / 1e9 / HBM_PEAK_GBS if el_ms > 0 else 0.0,
         "step_frac": step_bytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+        # its binding limit: VALU issue (counter-measured instructions per
+        # element over the launch time, against the nominal-clock issue peak)
+        "element_valu_issue_frac": (FP64_VALU_INSTS_PER_ELEMENT * prob.num_elem / 64 / (el_ms * 1e-3)
+                                    / VALU_ISSUE_PEAK_PER_S) if el_ms > 0 else 0.0,
     }
 
 
