@@ -9,7 +9,7 @@ before timing starts.
 
     python bench.py [--gpus 1] [--steps 500] [--warmup 20] [--precision mixed|fp64]
                     [--workload cube128_2blk] [--cpu-steps 40] [--no-cpu] [--uq NE,NA]
-                    [--no-fp64] [--fp64-steps 20] [--no-ensemble]
+                    [--no-fp64] [--fp64-steps 30] [--no-ensemble]
 
 Prints one JSON line.  `value` is element-steps/s of the timed K steps.
 `roofline` covers the dominant (element) kernel: its algorithmic bytes per
@@ -437,7 +437,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--uq", default="", metavar="NE,NA", help="add NE exact and NA approximate UQ samples")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 leg")
-    ap.add_argument("--fp64-steps", type=int, default=20)
+    ap.add_argument("--fp64-steps", type=int, default=30)
     ap.add_argument("--no-ensemble", action="store_true", help="skip the batched-ensemble leg")
     ap.add_argument("--no-surrogate", action="store_true", help="skip the learned-control-variate leg")
     ap.add_argument("--only", choices=["ensemble", "surrogate"], default=None,
@@ -492,7 +492,7 @@ def main():
     # the other GPU legs before the headline's steps (see the module notes)
     legs = {}
     if not args.no_fp64 and args.precision == "mixed" and not args.uq:
-        legs["fp64"] = fp64_leg(N, args.workload, args.fp64_steps, 5)
+        legs["fp64"] = fp64_leg(N, args.workload, args.fp64_steps, 20)
     if not args.no_ensemble and not args.uq:
         legs["ensemble"] = ensemble_leg(N)
     if not args.no_surrogate and not args.uq:
