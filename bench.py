@@ -443,6 +443,8 @@ def main():
     ap.add_argument("--only", choices=["ensemble", "surrogate"], default=None,
                     help="run only that leg and print its object (for profiling)")
     ap.add_argument("--surrogate-iterations", type=int, default=3000)
+    ap.add_argument("--hbm-probe", type=int, default=0, metavar="MIB",
+                    help="only time device fills and copies of MIB MiB (the practical HBM bandwidth)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run 5 device copies of MIB MiB (calibrates rocprofv3 FETCH_SIZE / WRITE_SIZE)")
     args = ap.parse_args()
@@ -469,6 +471,30 @@ def main():
             dst.copy_(src)
         torch.cuda.synchronize()
         print(json.dumps({"copy_calibration_bytes": nbytes, "copies": 5}), flush=True)
+        return
+
+    if args.hbm_probe:
+        # practical ceilings under the nominal 8 TB/s: a write-only stream
+        # (fill), a read-only stream (sum) and a read+write stream (copy),
+        # each the median of 20 timed repetitions after 3 untimed ones
+        nbytes = args.hbm_probe << 20
+        a = torch.empty(nbytes // 4, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        a.fill_(1.0)
+        res = {}
+        for name, fn, moved in (("fill", lambda: a.fill_(2.0), nbytes), ("read", lambda: a.sum(), nbytes),
+                                ("copy", lambda: b.copy_(a), 2 * nbytes)):
+            ts = []
+            for i in range(23):
+                e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                e0.record()
+                fn()
+                e1.record()
+                e1.synchronize()
+                if i >= 3:
+                    ts.append(e0.elapsed_time(e1) * 1e-3)
+            res[name + "_GBs"] = moved / float(np.median(ts)) / 1e9
+        print(json.dumps({"hbm_probe_bytes": nbytes, **res, "peak_GBs": HBM_PEAK_GBS}), flush=True)
         return
 
     if args.only == "ensemble":
