@@ -703,7 +703,7 @@ void free_device(nsm_ensemble* es)
 int issue_run(nsm_ensemble* es, int64_t n, int64_t output_frequency)
 {
   const bool     mixed = es->precision == NSM_MIXED;
-  nsm::EnsStep   st0{es->d_times, n + 1, 0, 0, 0, 0, 1};
+  nsm::EnsStep   st0{es->d_times, n + 1, 0, 0, 0, 0, 1, 1};
   nsm::launch_ens_reset(es->en, es->d_node_chunks, es->num_node_chunks, st0, es->stream);
   ENS_LAUNCHED();
   nsm::launch_ens_prekick(es->en, es->d_node_chunks, es->num_node_chunks, st0, es->stream);
@@ -717,8 +717,10 @@ int issue_run(nsm_ensemble* es, int64_t n, int64_t output_frequency)
   na.flags = es->en.flags;
   for (int64_t step = 0; step < n; step++) {
     const bool   last = step + 1 == n;
+    // a sample's internal force is read only after the run: stored on the
+    // last step (as the acceleration), not on every step
     nsm::EnsStep es_step{es->d_times, n + 1, (int)step, (int)is_output(step, n, output_frequency), (int)!last,
-                         (int)last, 1};
+                         (int)last, 1, (int)last};
     nsm::ReactionArgs rx{-1, nullptr};
     if (es->have_reaction) rx = nsm::ReactionArgs{es->rx_coord, es->d_rx_partials + (step & 1) * es->rx_stride};
     nsm::EnsFold fold{es->d_rx_partials + ((step + 1) & 1) * es->rx_stride, es->d_fold_ptr, es->d_fold_idx,

@@ -78,6 +78,7 @@ struct EnsStep {
   int64_t          times_stride;
   int              step;
   int              output_bc, fuse, write_a, update_v;
+  int              write_f;  // store the internal force (a run's last step)
 };
 
 // StepArgs of step es.step for sample s (s uniform).
@@ -90,6 +91,7 @@ __device__ __forceinline__ StepArgs ens_step_args(const EnsStep& es, int s)
   sa.output_bc = es.output_bc;
   sa.fuse      = es.fuse;
   sa.write_a   = es.write_a;
+  sa.write_f   = es.write_f;
   sa.update_v  = es.update_v;
   return sa;
 }

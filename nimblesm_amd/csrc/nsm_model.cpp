@@ -1252,7 +1252,7 @@ int nsm_model_compute_internal_force(nsm_model* md, double time_previous, double
   int st = ensure_device(md);
   if (st) return st;
   const nsm::StepTimes none{0.0, 0.0, 0.0, 0.0};
-  const nsm::StepArgs  sa{none, none, 0, 0, 0, 0};
+  const nsm::StepArgs  sa{none, none, 0, 0, 0, 0, 1};
   int64_t              pending = -1;
   if ((st = launch_step(md, sa, -1, -1, &pending))) return st;
   return check_status(md);
@@ -1284,7 +1284,7 @@ int nsm_model_compute_internal_force_from_stress(nsm_model* md)
   int st = check_split(md);
   if (st || (st = launch_elements(md, -1, nsm::kElementForcesFromStress))) return st;
   const nsm::StepTimes none{0.0, 0.0, 0.0, 0.0};
-  const nsm::StepArgs  sa{none, none, 0, 0, 0, 0};
+  const nsm::StepArgs  sa{none, none, 0, 0, 0, 0, 1};
   int64_t              pending = -1;
   if ((st = launch_node(md, sa, -1, -1, &pending, -1))) return st;
   return check_status(md);
@@ -1488,7 +1488,8 @@ int nsm_model_explicit_steps(nsm_model* md, int64_t step_begin, int64_t step_end
     double         tn   = t;
     nsm::StepTimes nxt  = cur;
     if (!last) nxt = step_times(&tn, dt_user);
-    const nsm::StepArgs sa{cur, nxt, (int)is_output(step, num_load_steps, output_frequency), (int)!last, (int)last, 1};
+    // the internal force is the step's entry-point output: stored every step
+    const nsm::StepArgs sa{cur, nxt, (int)is_output(step, num_load_steps, output_frequency), (int)!last, (int)last, 1, 1};
     // the previous step's pending reaction total is summed inside this launch
     if ((st = launch_step(md, sa, step, pending, &pending))) return st;
     if (!last) {

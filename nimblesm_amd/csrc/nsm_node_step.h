@@ -17,6 +17,7 @@ struct StepArgs {
   int       fuse;       // also perform the next step's first half
   int       write_a;    // store the acceleration
   int       update_v;   // 0: only store the internal force
+  int       write_f;    // store the internal force (always when update_v is 0)
 };
 
 // ApplyKinematicBC for the explicit scheme
@@ -80,7 +81,7 @@ __device__ __forceinline__ void node_update(const NodeArrays& na, const BcEntry*
                                             const StepArgs& sa)
 {
 #pragma clang fp contract(off)
-  store3_stream(na.f, i, f);
+  if (sa.write_f || !sa.update_v) store3_stream(na.f, i, f);
   if (!sa.update_v) return;
   const int    code     = flags & 0x7f;
   const double oneOverM = 1.0 / m;
