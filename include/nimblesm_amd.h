@@ -399,7 +399,9 @@ int nsm_ensemble_run(nsm_ensemble* ens, int64_t num_steps, int64_t output_freque
 int nsm_ensemble_sample_reaction_history(nsm_ensemble* ens, int num_points, double* out);
 int nsm_ensemble_get_reaction_history(nsm_ensemble* ens, int sample, int64_t first_step, int64_t count, double* out);
 /* Host copy of a sample's node field (NSM_FIELD_DISPLACEMENT, _VELOCITY,
- * _ACCELERATION, _INTERNAL_FORCE, _LUMPED_MASS) in its mesh's node order. */
+ * _ACCELERATION, _INTERNAL_FORCE, _LUMPED_MASS) in its mesh's node order;
+ * acceleration and internal force are those of the latest run's last step
+ * (the only step that stores them). */
 int nsm_ensemble_get_field(nsm_ensemble* ens, int sample, int field, double* out);
 /* Samples, nodes and elements over all samples, element tiles of all samples
  * (mixed; element chunks in FP64), the tile size in use, device bytes. */
