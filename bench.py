@@ -75,6 +75,9 @@ COUNTED_FLOPS_PER_ELEMENT_STEP = 9840
 # against: 1024 SIMDs, one wave64 FP64 instruction per 4 cycles, 2.4 GHz
 FP64_VALU_INSTS_PER_ELEMENT = 8 * 1897
 VALU_ISSUE_PEAK_PER_S = 1024 * 2.4e9 / 4
+# the same for the tiled mixed element kernel (one lane per element; SQ_INSTS_VALU
+# per wave of 64 elements on the 128^3 cube, 512-element tiles)
+MIXED_VALU_INSTS_PER_ELEMENT = 2388
 
 
 MAX_BLOCKS_PER_LAUNCH = 8  # element blocks sharing one element-kernel launch
@@ -613,6 +616,12 @@ def main():
             "compulsory_floor_bytes_per_element": floor_bytes / prob.num_elem,
             "compulsory_floor_frac": floor_bytes / prob.num_elem * value / (HBM_PEAK_GBS * 1e9),
             "element_measured_hbm_GBs": traffic / (el_ms * 1e-3) / 1e9 if traffic and el_ms > 0 else None,
+            # counter-measured VALU instructions over the launch time against the
+            # 2.4 GHz issue peak (packed FP32 is not used, so valu_frac's packed
+            # peak overstates what the element arithmetic could reach)
+            "element_valu_issue_frac": (MIXED_VALU_INSTS_PER_ELEMENT * prob.num_elem / 64 / (el_ms * 1e-3)
+                                        / VALU_ISSUE_PEAK_PER_S)
+            if args.precision == "mixed" and assembly["tiled"] and el_ms > 0 else None,
         },
     }
     if args.uq:
