@@ -82,28 +82,7 @@ using namespace fp64;
 // a = sum_j x_j (x) dN_j/dxi at ipt q with x = X + u, and b = sum_j X_j (x)
 // dN_j/dxi (src/nimble_element.h:442-482, which forms both in one loop; each
 // entry is its own chain of products and sums, so forming them apart gives
-// the same bits).  xs/us are [8][3].
-__device__ __forceinline__ void current_jacobian(int q, const double (*xs)[3], const double (*us)[3], double a[9])
-{
-  for (int i = 0; i < 9; i++) a[i] = 0.0;
-#pragma unroll
-  for (int j = 0; j < 8; j++) {
-    const double cc1 = xs[j][0] + us[j][0];
-    const double cc2 = xs[j][1] + us[j][1];
-    const double cc3 = xs[j][2] + us[j][2];
-    const double s1 = c_hex.dN[24 * q + 3 * j], s2 = c_hex.dN[24 * q + 3 * j + 1], s3 = c_hex.dN[24 * q + 3 * j + 2];
-    a[0] += cc1 * s1;
-    a[1] += cc1 * s2;
-    a[2] += cc1 * s3;
-    a[3] += cc2 * s1;
-    a[4] += cc2 * s2;
-    a[5] += cc2 * s3;
-    a[6] += cc3 * s1;
-    a[7] += cc3 * s2;
-    a[8] += cc3 * s3;
-  }
-}
-
+// the same bits).  xs/us are [8][3]: b alone, then both.
 __device__ __forceinline__ void reference_jacobian(int q, const double (*xs)[3], double b[9])
 {
   for (int i = 0; i < 9; i++) b[i] = 0.0;
@@ -134,6 +113,58 @@ __device__ __forceinline__ void jacobians(int q, const double (*xs)[3], const do
     const double cc1 = rc1 + us[j][0];
     const double cc2 = rc2 + us[j][1];
     const double cc3 = rc3 + us[j][2];
+    const double s1 = c_hex.dN[24 * q + 3 * j], s2 = c_hex.dN[24 * q + 3 * j + 1], s3 = c_hex.dN[24 * q + 3 * j + 2];
+    a[0] += cc1 * s1;
+    a[1] += cc1 * s2;
+    a[2] += cc1 * s3;
+    a[3] += cc2 * s1;
+    a[4] += cc2 * s2;
+    a[5] += cc2 * s3;
+    a[6] += cc3 * s1;
+    a[7] += cc3 * s2;
+    a[8] += cc3 * s3;
+    b[0] += rc1 * s1;
+    b[1] += rc1 * s2;
+    b[2] += rc1 * s3;
+    b[3] += rc2 * s1;
+    b[4] += rc2 * s2;
+    b[5] += rc2 * s3;
+    b[6] += rc3 * s1;
+    b[7] += rc3 * s2;
+    b[8] += rc3 * s3;
+  }
+}
+
+// The same two Jacobians from the current coordinates x = X + u formed once
+// per node (xc), as the element kernels stage them: each node's sum is the
+// one the functions above form in every lane, so the bits are the same.
+__device__ __forceinline__ void current_jacobian_x(int q, const double (*xc)[3], double a[9])
+{
+  for (int i = 0; i < 9; i++) a[i] = 0.0;
+#pragma unroll
+  for (int j = 0; j < 8; j++) {
+    const double cc1 = xc[j][0], cc2 = xc[j][1], cc3 = xc[j][2];
+    const double s1 = c_hex.dN[24 * q + 3 * j], s2 = c_hex.dN[24 * q + 3 * j + 1], s3 = c_hex.dN[24 * q + 3 * j + 2];
+    a[0] += cc1 * s1;
+    a[1] += cc1 * s2;
+    a[2] += cc1 * s3;
+    a[3] += cc2 * s1;
+    a[4] += cc2 * s2;
+    a[5] += cc2 * s3;
+    a[6] += cc3 * s1;
+    a[7] += cc3 * s2;
+    a[8] += cc3 * s3;
+  }
+}
+
+__device__ __forceinline__ void jacobians_x(int q, const double (*xs)[3], const double (*xc)[3], double a[9],
+                                            double b[9])
+{
+  for (int i = 0; i < 9; i++) a[i] = b[i] = 0.0;
+#pragma unroll
+  for (int j = 0; j < 8; j++) {
+    const double rc1 = xs[j][0], rc2 = xs[j][1], rc3 = xs[j][2];
+    const double cc1 = xc[j][0], cc2 = xc[j][1], cc3 = xc[j][2];
     const double s1 = c_hex.dN[24 * q + 3 * j], s2 = c_hex.dN[24 * q + 3 * j + 1], s3 = c_hex.dN[24 * q + 3 * j + 2];
     a[0] += cc1 * s1;
     a[1] += cc1 * s2;
@@ -205,8 +236,8 @@ __device__ __forceinline__ void node_force(int n, const double (*ai)[10], const 
 struct Lds {
   union {
     struct {
-      double x[kEPW][8][3];
-      double u[kEPW][8][3];
+      double x[kEPW][8][3];   // reference coordinates X
+      double xc[kEPW][8][3];  // current coordinates X + u
     } nodes;
     struct {
       double ai[kEPW][8][10];
@@ -252,9 +283,9 @@ __device__ __forceinline__ void element_fp64(Lds& lds, int64_t e, bool valid, do
     lds.nodes.x[el][k][0] = X[3 * nd];
     lds.nodes.x[el][k][1] = X[3 * nd + 1];
     lds.nodes.x[el][k][2] = X[3 * nd + 2];
-    lds.nodes.u[el][k][0] = u[3 * nd];
-    lds.nodes.u[el][k][1] = u[3 * nd + 1];
-    lds.nodes.u[el][k][2] = u[3 * nd + 2];
+    lds.nodes.xc[el][k][0] = lds.nodes.x[el][k][0] + u[3 * nd];
+    lds.nodes.xc[el][k][1] = lds.nodes.x[el][k][1] + u[3 * nd + 1];
+    lds.nodes.xc[el][k][2] = lds.nodes.x[el][k][2] + u[3 * nd + 2];
   }
   __syncthreads();
 
@@ -263,10 +294,10 @@ __device__ __forceinline__ void element_fp64(Lds& lds, int64_t e, bool valid, do
   if (valid) {
     double a[9];
     if (kRefTable || kMode == kElementForcesFromStress) {
-      current_jacobian(k, lds.nodes.x[el], lds.nodes.u[el], a);
+      current_jacobian_x(k, lds.nodes.xc[el], a);
     } else {
       double b[9];
-      jacobians(k, lds.nodes.x[el], lds.nodes.u[el], a, b);
+      jacobians_x(k, lds.nodes.x[el], lds.nodes.xc[el], a, b);
       detb = invert3x3<false>(b, bi);
     }
     if (kMode != kElementForcesFromStress) {
@@ -418,14 +449,14 @@ __global__ __launch_bounds__(kThreads) void k_seam_forces(int64_t E, const doubl
   const int64_t e  = (int64_t)blockIdx.x * kEPW + el;
   if (e < E)
     for (int c = 0; c < 3; c++) {
-      lds.nodes.x[el][k][c] = Xg[(e * 8 + k) * 3 + c];
-      lds.nodes.u[el][k][c] = ug[(e * 8 + k) * 3 + c];
+      lds.nodes.x[el][k][c]  = Xg[(e * 8 + k) * 3 + c];
+      lds.nodes.xc[el][k][c] = Xg[(e * 8 + k) * 3 + c] + ug[(e * 8 + k) * 3 + c];
     }
   __syncthreads();
   double ainv[10];
   if (e < E) {
     double a[9], b[9];
-    jacobians(k, lds.nodes.x[el], lds.nodes.u[el], a, b);
+    jacobians_x(k, lds.nodes.x[el], lds.nodes.xc[el], a, b);
     ainv[9] = invert3x3(a, ainv);
     if (ainv[9] <= 0.0) atomicOr(status, 1);
   }
