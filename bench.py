@@ -27,15 +27,12 @@ error of the mixed ensemble against the FP64 ensemble, and plain
 one-at-a-time solves of the model path (create, lumped mass and dt, 400
 steps, QoI) timed on two samples per edge count.
 
-Order of the legs: the headline model is built first, then the FP64,
-ensemble and surrogate legs run, and only then the headline's warmup and
-timed steps, with the CPU baseline last.  A GPU coming out of idle runs its
-first steps slower while its clocks rise (20-step chunks of the headline
-step after a 5-step warmup from idle: 0.276, 0.254, then 0.245 ms/step;
-after 200 ms of other device work the first chunk is already 0.252), so a
-short timed run placed after host-side setup would measure that ramp rather
-than the step.  The headline's own warmup and timed steps are exactly the
-requested counts.
+Short timed runs: a GPU coming out of idle runs its first steps slower while
+its clocks rise (20-step chunks of the headline step after a 5-step warmup
+from idle: 0.276, 0.254, then 0.245 ms/step on one box), so a run of 20 timed
+steps reads a few percent slower than one of 500.  The timed steps continue
+from the warmed state without a host reset, which would idle the device
+between them.
 
 A `surrogate` object covers the learned control variate on the 24^3 cube:
 512 plain FP64 solves (batched) train the 4-256-256-256-64 network on the
@@ -518,15 +515,6 @@ def main():
         model.SetUqSamples(*uq)
 
     # warm-up run, then restart from rest for the timed run of K steps
-    # the other GPU legs before the headline's steps (see the module notes)
-    legs = {}
-    if not args.no_fp64 and args.precision == "mixed" and not args.uq:
-        legs["fp64"] = fp64_leg(N, args.workload, args.fp64_steps, 20)
-    if not args.no_ensemble and not args.uq:
-        legs["ensemble"] = ensemble_leg(N)
-    if not args.no_surrogate and not args.uq:
-        legs["surrogate"] = surrogate_leg(N, iterations=args.surrogate_iterations)
-
     # warmup steps, then the timed steps continue from that state (no host
     # reset between them: it would leave the device idle)
     integ = prob.make_integrator(model, num_steps=max(total, 1))
@@ -635,7 +623,12 @@ def main():
             "closure_frac": cb / (cl_ms * 1e-3) / 1e9 / HBM_PEAK_GBS if cl_ms > 0 else 0.0,
         }
     model.close()
-    out.update(legs)
+    if not args.no_fp64 and args.precision == "mixed" and not args.uq:
+        out["fp64"] = fp64_leg(N, args.workload, args.fp64_steps, 20)
+    if not args.no_ensemble and not args.uq:
+        out["ensemble"] = ensemble_leg(N)
+    if not args.no_surrogate and not args.uq:
+        out["surrogate"] = surrogate_leg(N, iterations=args.surrogate_iterations)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(prob, args.cpu_steps)
     print(json.dumps(out), flush=True)
