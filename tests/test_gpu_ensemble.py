@@ -5,7 +5,9 @@ that sample alone computes: bit for bit in both precisions (same element
 arithmetic, same summation orders, same reaction sums), including the lumped
 mass and the critical time step, which the ensemble computes per sample.  The
 single-model FP64 path is itself bit-identical to the oracle (test_gpu_model);
-one sample is also checked against the oracle directly here.  Graph replays
+one sample is also checked against the oracle directly here.  Besides
+jittered cubes, the meshes of three reference decks (one unstructured, one
+with two blocks) run side by side in one ensemble.  Graph replays
 must equal eager runs, also after the samples' parameters change.  The full
 ensemble workload (256 samples on the 32^3 cube and 64 on uneven cubes, 400
 steps) is checked in mixed precision against the same ensemble in FP64.
@@ -237,4 +239,78 @@ def test_displacement_and_initial_velocity_conditions(hip_device, precision):
             m.close()
     finally:
         del os.environ["NSM_TILE_ELEMS"]
+    ens.close()
+
+
+def deck_conditions(g):
+    """The golden deck's conditions as (kind, nodes, coordinate, value)
+    groups of consecutive entries with one kind, coordinate and value."""
+    kinds = ("initial_velocity", "prescribed_velocity", "prescribed_displacement")
+    key = list(zip(g["bc_kind"], g["bc_coord"], g["bc_value"]))
+    out, i = [], 0
+    while i < len(key):
+        j = i
+        while j < len(key) and key[j] == key[i]:
+            j += 1
+        kind, c, v = key[i]
+        out.append((kinds[int(kind)], g["bc_node"][i:j].astype(np.int32), int(c), float(v)))
+        i = j
+    return out
+
+
+@pytest.mark.parametrize("precision,tile", [("fp64", 0), ("mixed", 64), ("mixed", 128)])
+def test_reference_deck_meshes_in_one_ensemble(hip_device, precision, tile):
+    """The meshes of three reference decks (wave_in_bar; brick_with_fibers,
+    two blocks; notched_plate, unstructured) in one ensemble, two samples
+    each: the deck's own parameters, and moduli 0.8x, density 1.1x with
+    condition magnitudes 0.7x.  Every sample is bit-identical to a ModelData
+    run of the same mesh, materials and conditions (deck time step, 40 steps,
+    output steps every 6)."""
+    from tests.conftest import load_golden
+    steps, of = 40, 6
+    names = ("wave_in_bar", "brick_with_fibers", "notched_plate_native_neohookean")
+    ens = N.Ensemble(precision, tile)
+    cases = []
+    for name in names:
+        g = load_golden(name)
+        blocks = [g["conn"][int(g["blk_off"][b]):int(g["blk_off"][b + 1])] for b in range(len(g["rho"]))]
+        conds = deck_conditions(g)
+        react = next(c for c in conds if c[0] == "prescribed_velocity")  # the held end, x reaction
+        mid = ens.AddMesh(g["X"], blocks)
+        for kind, nodes, c, _ in conds:
+            ens.AddBoundaryCondition(mid, kind, nodes, c)
+        ens.SetReactionNodes(mid, react[1], react[2])
+        rho, K, G = (np.asarray(g[k], np.float64) for k in ("rho", "K", "G"))
+        mags = np.array([v for *_, v in conds])
+        params = [(rho, K, G, mags), (1.1 * rho, 0.8 * K, 0.8 * G, 0.7 * mags)]
+        ens.AddSamples(mid, [p[0] for p in params], [p[1] for p in params], [p[2] for p in params],
+                       [p[3] for p in params])
+        dt = float(g["t_final"]) / int(g["num_load_steps"])
+        cases += [(g, blocks, conds, react, p, dt) for p in params]
+    ens.Setup()
+    dts = ens.CriticalTimeSteps()
+    ens.Run(steps, np.array([c[-1] * steps for c in cases]), output_frequency=of, use_graph=True)
+    old = os.environ.get("NSM_TILE_ELEMS")
+    os.environ["NSM_TILE_ELEMS"] = str(tile or 512)
+    try:
+        for s, (g, blocks, conds, react, (rho, K, G, mags), dt) in enumerate(cases):
+            m = N.ModelData(g["X"], [N.Block(int(g["blk_ids"][b]), blocks[b],
+                                             N.NeohookeanMaterial(rho[b], K[b], G[b])) for b in range(len(blocks))],
+                            precision)
+            for (kind, nodes, c, _), v in zip(conds, mags):
+                m.AddBoundaryCondition(kind, nodes, c, float(v))
+            m.SetReactionNodes(react[1], react[2])
+            integ = N.ExplicitTimeIntegrator(m, dt * steps, steps, of).Initialize()
+            assert integ.critical_time_step == dts[s], s
+            integ.Integrate()
+            assert np.array_equal(ens.GetNodeData(s, "lumped_mass"), m.GetScalarNodeData("lumped_mass")), s
+            for f in FIELDS:
+                assert np.array_equal(ens.GetNodeData(s, f), m.GetVectorNodeData(f)), (s, f)
+            assert np.array_equal(ens.GetReactionHistory(s), m.GetReactionHistory(0, steps)), s
+            m.close()
+    finally:
+        if old is None:
+            del os.environ["NSM_TILE_ELEMS"]
+        else:
+            os.environ["NSM_TILE_ELEMS"] = old
     ens.close()
