@@ -230,9 +230,11 @@ __device__ __forceinline__ void gemm_tile(const GemmArgs& g, int mode, int bx, i
   }
 }
 
+// Row tiles run along the grid's x dimension (up to 2^31 - 1 workgroups; y
+// and z are limited to 65535), column tiles along y, K ranges along z.
 __global__ __launch_bounds__(kGemmThreads) void k_gemm_bf16(GemmArgs g, int mode)
 {
-  gemm_tile(g, mode, (int)blockIdx.x, (int)blockIdx.y, (int)blockIdx.z);
+  gemm_tile(g, mode, (int)blockIdx.y, (int)blockIdx.x, (int)blockIdx.z);
 }
 
 // db[n] = sum over the batch of G[b][n] for the four columns 4 w .. 4 w + 3
@@ -434,7 +436,7 @@ int mlp_gemm(nsm_mlp* net, GemmArgs a, int mode, int splits = 1)
     a.c            = net->split;
     a.split_stride = mn;
   }
-  const dim3 grid(blocks_of(a.N, kTile), blocks_of(a.M, kTile), (unsigned)splits);
+  const dim3 grid(blocks_of(a.M, kTile), blocks_of(a.N, kTile), (unsigned)splits);
   hipLaunchKernelGGL(k_gemm_bf16, grid, dim3(kGemmThreads), 0, net->stream, a, mode);
   MLP_LAUNCHED();
   if (splits > 1) {

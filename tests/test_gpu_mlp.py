@@ -44,6 +44,27 @@ def test_single_layer_is_an_exact_bf16_gemm(hip_device, K, N, B):
     net.close()
 
 
+def test_batch_past_65535_row_tiles(hip_device):
+    """2,097,192 rows: more 32-row tiles than a grid's y or z dimension may
+    hold; rows from every part of the batch, the last ones included, are
+    the exact BF16 product."""
+    K, N, B = 4, 16, 65536 * 32 + 40
+    net = MLP((K, N), seed=5)
+    rng = np.random.default_rng(8)
+    p = net.get_parameters()
+    p[-N:] = rng.normal(0, 1, N).astype(np.float32)
+    net.set_parameters(p)
+    x = rng.normal(0, 1, (B, K)).astype(np.float32)
+    y = net.forward(x).astype(np.float64)
+    rows = np.concatenate([np.arange(64), rng.choice(B, 2000, replace=False), np.arange(B - 100, B)])
+    W = M.to_bf16(p[:N * K].reshape(N, K))
+    X = M.to_bf16(x[rows])
+    ref = X @ W.T + p[N * K:].astype(np.float64)
+    bound = 2 * K * U * (np.abs(X) @ np.abs(W).T) + 4 * U * np.abs(ref)
+    assert np.all(np.abs(y[rows] - ref) <= bound), np.max(np.abs(y[rows] - ref) / bound)
+    net.close()
+
+
 def test_initial_parameters_match_the_oracle(hip_device):
     widths = (4, 256, 256, 256, 64)
     net = MLP(widths, seed=12345)
