@@ -514,7 +514,6 @@ def main():
         uq = uq_ensemble(prob, ne, na)
         model.SetUqSamples(*uq)
 
-    # warm-up run, then restart from rest for the timed run of K steps
     # warmup steps, then the timed steps continue from that state (no host
     # reset between them: it would leave the device idle)
     integ = prob.make_integrator(model, num_steps=max(total, 1))
@@ -587,6 +586,7 @@ def main():
             "element_launches_per_step": launches_per_step,
             "element_alg_bytes_per_launch": bytes_per_launch,
             "node_ms_avg": nd_ms, "node_launches_timed": nd_n,
+            "tile_elems": model.TileElems() if assembly["tiled"] else 0,
             "tiles": assembly["num_tiles"], "interior_nodes": assembly["interior_nodes"],
             "boundary_nodes": assembly["boundary_nodes"],
             "reaction_ms_avg": rx_ms, "reaction_launches": rx_n,

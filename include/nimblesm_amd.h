@@ -265,7 +265,7 @@ int nsm_model_assembly_info(nsm_model* model, int* tiled, int64_t* num_tiles, in
                             int64_t* boundary_nodes, int64_t* gather_tiles);
 /* Elements per element tile (= threads per workgroup of the tiled kernel) of
  * a tiled mixed model, 0 otherwise.  Chosen at creation: NSM_TILE_ELEMS (64,
- * 128, 256, 512) when set, 512 when that gives every CU two tiles, else by
+ * 128, 256, 512) when set, 512 when that gives every CU eight tiles, else by
  * the autotuner from a few timed steps with each size (cached per mesh shape
  * for the process, and in the file NSM_AUTOTUNE_CACHE names).  No reference
  * counterpart (the reference's Kokkos policies are fixed,

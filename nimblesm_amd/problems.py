@@ -123,6 +123,8 @@ WORKLOADS = {
     "cube16_fp64": dict(n=16, materials=[STEEL], num_steps=1000),
     "cube128_2blk": dict(n=128, materials=[STEEL, ALUMINIUM], num_steps=500),
     "cube256_4blk": dict(n=256, materials=[STEEL, ALUMINIUM, TITANIUM, POLYMER], num_steps=200),
+    # one GPU's share of cube128_2blk split into 2 x 2 x 2 bricks: a 64^3 brick of one block
+    "cube64_1blk": dict(n=64, materials=[STEEL], num_steps=500),
 }
 
 
