@@ -235,6 +235,42 @@ def fp64_leg(N, workload, steps, warmup):
     }
 
 
+def host_seam_leg(N, workload, calls=10):
+    """The internal-force seam driven with host buffers, as a caller that
+    keeps its own integrator loop on the host would drive it
+    (ModelDataBase::ComputeInternalForce takes host views of u and f): per
+    call, u goes to the device, the force evaluation runs, f comes back.
+    Reported beside the same evaluation on device-resident data."""
+    prob = N.workload(workload).build()
+    rng = np.random.default_rng(3)
+    u = 1e-4 / prob.n * rng.uniform(-1.0, 1.0, (prob.num_nodes, 3))
+    out = {"calls": calls, "bytes_per_call": 2 * u.nbytes}
+    for precision in ("mixed", "fp64"):
+        model = prob.make_model(precision)
+        model.ComputeLumpedMass()
+        model.SetVectorNodeData("displacement", u)
+        model.ComputeInternalForce()
+        f = model.GetVectorNodeData("internal_force")
+        t0 = time.perf_counter()
+        for _ in range(calls):
+            model.SetVectorNodeData("displacement", u)
+            model.ComputeInternalForce()
+            f = model.GetVectorNodeData("internal_force")
+        seam = (time.perf_counter() - t0) / calls
+        model.Synchronize()
+        t0 = time.perf_counter()
+        for _ in range(calls):
+            model.ComputeInternalForce()
+        model.Synchronize()
+        dev = (time.perf_counter() - t0) / calls
+        assert np.isfinite(f).all()
+        model.close()
+        out[precision] = {"ms_per_call": seam * 1e3, "device_ms_per_call": dev * 1e3,
+                          "element_evaluations_per_s": prob.num_elem / seam,
+                          "device_element_evaluations_per_s": prob.num_elem / dev}
+    return out
+
+
 def ensemble_leg(N, num_steps=400, runs=3, plain_per_size=2):
     """Batched ensemble of 256 samples on the 32^3 cube and 64 on uneven
     cubes (edge counts from {16, 24, 32, 40, 48}), num_steps steps each:
@@ -440,6 +476,7 @@ def main():
     ap.add_argument("--fp64-steps", type=int, default=30)
     ap.add_argument("--no-ensemble", action="store_true", help="skip the batched-ensemble leg")
     ap.add_argument("--no-surrogate", action="store_true", help="skip the learned-control-variate leg")
+    ap.add_argument("--no-seam", action="store_true", help="skip the host-buffer internal-force seam leg")
     ap.add_argument("--only", choices=["ensemble", "surrogate"], default=None,
                     help="run only that leg and print its object (for profiling)")
     ap.add_argument("--surrogate-iterations", type=int, default=3000)
@@ -629,6 +666,8 @@ def main():
         out["ensemble"] = ensemble_leg(N)
     if not args.no_surrogate and not args.uq:
         out["surrogate"] = surrogate_leg(N, iterations=args.surrogate_iterations)
+    if not args.no_seam and not args.uq:
+        out["host_seam"] = host_seam_leg(N, args.workload)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(prob, args.cpu_steps)
     print(json.dumps(out), flush=True)

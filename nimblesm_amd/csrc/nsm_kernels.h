@@ -189,5 +189,10 @@ void launch_uq_closure(const NodeArrays& na, const UqClosure& uc, const StepArgs
 void launch_sample_history(const double* hist, int64_t n, int num_samples, double* out, hipStream_t s);
 // out = -sum_k partials[tiles[k]], k ascending (fixed tree, reproducible)
 void launch_reaction_finalize(const double* partials, const int* tiles, int num_tiles, double* out, hipStream_t s);
+// node field of width w between mesh order and a model's device order:
+// to_device: dst[map[n] w + c] = src[n w + c]; else dst[n w + c] = src[map[n] w + c]
+// (map[n] = device index of mesh node n, a permutation of [0, N))
+void launch_permute_nodes(const double* src, const int32_t* map, int64_t N, int w, bool to_device, double* dst,
+                          hipStream_t s);
 
 }  // namespace nsm

@@ -175,6 +175,11 @@ struct nsm_model {
   std::vector<int>     blk_tile_off;  // first tile of each block, [num_blocks + 1]
   nsm::TileArgs        tile_args{};
   std::vector<void*>   tile_bufs;
+  // host transfers of node fields: mesh -> device node map and a staging
+  // array in mesh order (allocated at the first transfer)
+  int32_t* d_node_new   = nullptr;
+  double*  d_stage      = nullptr;
+  size_t   stage_values = 0;
 
   // off-nominal ensemble (UQ): uq_exact exact samples, then uq_approx
   // approximate ones; sample s owns u, v, a, f at uq_state + (4 s + j) 3N
@@ -715,23 +720,49 @@ bool is_output(int64_t step, int64_t num_load_steps, int64_t output_frequency)
 }
 
 // device node array with w values per node -> host array in mesh node order
-int node_field_out(nsm_model* md, const double* src, size_t w, double* out)
+// device map and staging array for node fields of width up to w (tiled models)
+int ensure_node_staging(nsm_model* md, size_t w)
 {
-  const size_t cnt = w * (size_t)md->N;
-  if (!md->tiled) {
-    NSM_HIP(hipMemcpyAsync(out, src, cnt * sizeof(double), hipMemcpyDeviceToHost, md->stream));
-    NSM_HIP(hipStreamSynchronize(md->stream));
-    return NSM_OK;
+  if (!md->d_node_new) {
+    int32_t* p = nullptr;
+    int      st = dalloc(&p, (size_t)md->N, &md->device_bytes);
+    if (st) return st;
+    md->d_node_new = p;
+    NSM_HIP(hipMemcpyAsync(md->d_node_new, md->tiles.node_new.data(), (size_t)md->N * sizeof(int32_t),
+                           hipMemcpyHostToDevice, md->stream));
   }
-  std::vector<double> tmp(cnt);
-  NSM_HIP(hipMemcpyAsync(tmp.data(), src, cnt * sizeof(double), hipMemcpyDeviceToHost, md->stream));
-  NSM_HIP(hipStreamSynchronize(md->stream));
-  for (int64_t n = 0; n < md->N; n++)
-    for (size_t c = 0; c < w; c++) out[(size_t)n * w + c] = tmp[(size_t)dev_node(md, n) * w + c];
+  const size_t need = w * (size_t)md->N;
+  if (md->stage_values < need) {
+    NSM_HIP(hipStreamSynchronize(md->stream));
+    free_dev(md->d_stage);
+    md->device_bytes -= std::min(md->device_bytes, md->stage_values * sizeof(double));
+    md->stage_values = 0;
+    int st = dalloc(&md->d_stage, need, &md->device_bytes);
+    if (st) return st;
+    md->stage_values = need;
+  }
   return NSM_OK;
 }
 
-// host array in mesh node order -> device node array
+// device node array -> host array in mesh node order; a tiled model's rows
+// are permuted on the device into the staging array first
+int node_field_out(nsm_model* md, const double* src, size_t w, double* out)
+{
+  const size_t cnt = w * (size_t)md->N;
+  if (md->tiled) {
+    int st = ensure_node_staging(md, w);
+    if (st) return st;
+    nsm::launch_permute_nodes(src, md->d_node_new, md->N, (int)w, false, md->d_stage, md->stream);
+    NSM_LAUNCHED();
+    src = md->d_stage;
+  }
+  NSM_HIP(hipMemcpyAsync(out, src, cnt * sizeof(double), hipMemcpyDeviceToHost, md->stream));
+  NSM_HIP(hipStreamSynchronize(md->stream));
+  return NSM_OK;
+}
+
+// host array in mesh node order -> device node array (through the staging
+// array and a device permutation for a tiled model)
 int node_field_in(nsm_model* md, double* dst, size_t w, const double* in)
 {
   const size_t cnt = w * (size_t)md->N;
@@ -740,10 +771,11 @@ int node_field_in(nsm_model* md, double* dst, size_t w, const double* in)
     NSM_HIP(hipStreamSynchronize(md->stream));
     return NSM_OK;
   }
-  std::vector<double> tmp(cnt);
-  for (int64_t n = 0; n < md->N; n++)
-    for (size_t c = 0; c < w; c++) tmp[(size_t)dev_node(md, n) * w + c] = in[(size_t)n * w + c];
-  NSM_HIP(hipMemcpyAsync(dst, tmp.data(), cnt * sizeof(double), hipMemcpyHostToDevice, md->stream));
+  int st = ensure_node_staging(md, w);
+  if (st) return st;
+  NSM_HIP(hipMemcpyAsync(md->d_stage, in, cnt * sizeof(double), hipMemcpyHostToDevice, md->stream));
+  nsm::launch_permute_nodes(md->d_stage, md->d_node_new, md->N, (int)w, true, dst, md->stream);
+  NSM_LAUNCHED();
   NSM_HIP(hipStreamSynchronize(md->stream));
   return NSM_OK;
 }
@@ -1126,6 +1158,8 @@ int nsm_model_destroy(nsm_model* md)
   free_dev(md->uq_state);
   free_dev(md->uq_coeff);
   free_dev(md->uq_bc_table);
+  free_dev(md->d_node_new);
+  free_dev(md->d_stage);
   for (void* p : md->tile_bufs) (void)hipFree(p);
   for (auto& p : md->event_pool) {
     (void)hipEventDestroy(p.start);

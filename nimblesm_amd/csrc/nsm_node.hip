@@ -175,6 +175,20 @@ __global__ __launch_bounds__(kNodeThreads) void k_sample_history(const double* _
   out[k] = sampled_history(hist, n, k, num_samples);
 }
 
+// One node per thread; its w values move as one row.
+__global__ __launch_bounds__(kNodeThreads) void k_permute_nodes(const double* __restrict__ src,
+                                                                const int32_t* __restrict__ map, int64_t N, int w,
+                                                                int to_device, double* __restrict__ dst)
+{
+  const int64_t n = (int64_t)blockIdx.x * kNodeThreads + threadIdx.x;
+  if (n >= N) return;
+  const int64_t d = map[n];
+  if (to_device)
+    for (int c = 0; c < w; c++) dst[d * w + c] = src[n * w + c];
+  else
+    for (int c = 0; c < w; c++) dst[n * w + c] = src[d * w + c];
+}
+
 // UQ closure (UqModel::ApplyClosure, src/uq/nimble_uq.cc:327-346) fused with
 // the approximate samples' node update: per node, f_s = f_nominal * c0, then
 // f_s += f_exact_k * c_{k+1} for k ascending, then node_step with the
@@ -411,6 +425,14 @@ void launch_ens_node_fp64(const EnsNodes& en, const EnsRange* chunks, int num_ch
   if (num_chunks <= 0) return;
   hipLaunchKernelGGL(k_ens_node_fp64, dim3((unsigned)num_chunks), dim3(kNodeThreads), 0, s, en, chunks, node_ptr,
                      node_ent, slots, es, rx, fold);
+}
+
+void launch_permute_nodes(const double* src, const int32_t* map, int64_t N, int w, bool to_device, double* dst,
+                          hipStream_t s)
+{
+  if (N <= 0) return;
+  hipLaunchKernelGGL(k_permute_nodes, dim3((unsigned)((N + kNodeThreads - 1) / kNodeThreads)), dim3(kNodeThreads), 0,
+                     s, src, map, N, w, (int)to_device, dst);
 }
 
 void launch_ens_fold(EnsFold fold, hipStream_t s)
