@@ -333,8 +333,8 @@ def sample_history(R, num_samples=64):
     """The 64-point quantity of interest: history R[0..n-1] (value after each
     step) sampled at p_k = (k+1) n / num_samples steps with linear
     interpolation between integer steps j, where step j >= 1 holds R[j-1]
-    and step 0 is 0 (at rest).  The rule is this build's own definition (the
-    reference has no such output); restated here as the checker."""
+    and step 0 is 0 (at rest).  The rule is defined with the device sampler
+    (the reference has no such output); restated here as the checker."""
     R = np.asarray(R, np.float64)
     n = len(R)
     full = np.concatenate([[0.0], R])  # value at integer step j

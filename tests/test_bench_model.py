@@ -1,6 +1,7 @@
 """The bench line's byte model (bench.py algorithmic_bytes) on the CPU: the
 per-element-step figures of the entry-point model at 128^3 (mixed 667.6 B,
-FP64 1147.6 B per element-step, SURVEY's tables), and the split over launches
+FP64 1147.6 B per element-step: E (32 + 120 s) + N 152 with s = 4 or 8
+bytes per Gauss-point word, over E), and the split over launches
 adds back up to the whole step (tiled: element launch + boundary launch;
 slots: element launches + node launch)."""
 import numpy as np

@@ -1,6 +1,6 @@
 """The 64-point quantity of interest: the reaction history sampled at
 equally spaced times (sample k at (k+1) n / 64 steps, linear between steps,
-0 at rest before step 0).  The rule is this build's own definition (the
+0 at rest before step 0).  The rule is defined with the device sampler (the
 reference has no reaction output); the oracle's restatement is pinned here by
 closed forms, the GPU sampler against it in tests/test_gpu_model.py."""
 import numpy as np
