@@ -14,7 +14,8 @@ before timing starts.
 Prints one JSON line.  `value` is element-steps/s of the timed K steps.
 `roofline` covers the dominant (element) kernel: its algorithmic bytes per
 launch (algorithmic_bytes below) over its average launch duration from HIP
-events on the model's stream around every 10th launch of the timed steps.  `cpu_baseline` times the FP64 CPU oracle (OpenMP, deterministic
+events that every 10th launch of the timed steps records at its own start
+and end (hipExtLaunchKernelGGL).  `cpu_baseline` times the FP64 CPU oracle (OpenMP, deterministic
 gather) on a bounded sample of the same workload.
 
 The same line carries an `fp64` object (the plain FP64 path on the same
@@ -78,7 +79,7 @@ MIXED_VALU_INSTS_PER_ELEMENT = 2388
 
 
 MAX_BLOCKS_PER_LAUNCH = 8  # element blocks sharing one element-kernel launch
-TIMING_EVERY = 10  # kernel launches bracketed by HIP events: every 10th of each kernel
+TIMING_EVERY = 10  # kernel launches timed by HIP events: every 10th of each kernel
 
 
 def algorithmic_bytes(prob, s_bytes, assembly):
@@ -556,9 +557,9 @@ def main():
     integ = prob.make_integrator(model, num_steps=max(total, 1))
     integ.Advance(args.warmup)
 
-    # HIP events bracket every TIMING_EVERY-th launch of each kernel inside
-    # the timed region (an event pair costs a few microseconds of stream
-    # time, so timing every launch would tax the measured throughput)
+    # every TIMING_EVERY-th launch of each kernel inside the timed region
+    # records a start and a stop event (timing every launch would cost a few
+    # microseconds per launch of the measured throughput)
     model.EnableKernelTiming(True, every=TIMING_EVERY)
     torch.cuda.synchronize()
     model.Synchronize()

@@ -685,19 +685,19 @@ inline unsigned grid_of(int64_t n, int64_t per) { return (unsigned)((n + per - 1
 
 void launch_element_fp64(const BlockTable<double>& bt, int64_t E_total, const int* conn, const double* X,
                          const double* u, const double* rjac, double* Fip, double* sip, double* slots, int* status,
-                         hipStream_t s, int mode)
+                         hipStream_t s, int mode, const LaunchTimer& timer)
 {
   const int64_t n = bt.off[bt.count] - bt.off[0];
-  if (n <= 0) return;
+  if (n <= 0) return complete_timer(timer, s);
   const dim3 grid(grid_of(n, kEPW)), block(kThreads);
-#define NSM_FP64_LAUNCH(MODE)                                                                                  \
-  do {                                                                                                         \
-    if (rjac)                                                                                                  \
-      hipLaunchKernelGGL((k_element_fp64<MODE, true>), grid, block, 0, s, bt, E_total, conn, X, u, rjac, Fip, \
-                         sip, slots, status);                                                                  \
-    else                                                                                                       \
-      hipLaunchKernelGGL((k_element_fp64<MODE, false>), grid, block, 0, s, bt, E_total, conn, X, u, rjac, Fip, \
-                         sip, slots, status);                                                                  \
+#define NSM_FP64_LAUNCH(MODE)                                                                                   \
+  do {                                                                                                          \
+    if (rjac)                                                                                                   \
+      hipExtLaunchKernelGGL((k_element_fp64<MODE, true>), grid, block, 0, s, timer.start, timer.stop, 0, bt,    \
+                            E_total, conn, X, u, rjac, Fip, sip, slots, status);                                \
+    else                                                                                                        \
+      hipExtLaunchKernelGGL((k_element_fp64<MODE, false>), grid, block, 0, s, timer.start, timer.stop, 0, bt,   \
+                            E_total, conn, X, u, rjac, Fip, sip, slots, status);                                \
   } while (0)
   if (mode == kElementDefGrad)
     NSM_FP64_LAUNCH(kElementDefGrad);

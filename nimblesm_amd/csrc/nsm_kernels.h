@@ -9,6 +9,7 @@
 //                         (element e, local node k), ascending e per node
 #pragma once
 
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -32,6 +33,21 @@ struct BcEntry {
 // to the reaction-force set
 constexpr uint8_t kReactionFlag = 0x80;
 constexpr int     kMaxBcEntries = 127;
+
+// Events a timed launch records itself (hipExtLaunchKernelGGL): the start
+// and end of the kernel's own execution, without the stream markers that an
+// event recorded before and after the launch would add.  Null: not timed.
+struct LaunchTimer {
+  hipEvent_t start = nullptr, stop = nullptr;
+};
+// a timed launch that has no work still completes its events (zero time)
+inline void complete_timer(const LaunchTimer& t, hipStream_t s)
+{
+  if (t.start) {
+    (void)hipEventRecord(t.start, s);
+    (void)hipEventRecord(t.stop, s);
+  }
+}
 
 // Times of one explicit step, computed on the host exactly as
 // src/integrators/explicit_time_integrator.cc:192-195 does.
@@ -122,13 +138,14 @@ constexpr int kElementFull = 0, kElementDefGrad = 1, kElementForcesFromStress = 
 // rjac: the reference inverse Jacobians of launch_reference_jacobians
 void launch_element_fp64(const BlockTable<double>& bt, int64_t E_total, const int* conn, const double* X,
                          const double* u, const double* rjac, double* Fip, double* sip, double* slots, int* status,
-                         hipStream_t s, int mode = kElementFull);
+                         hipStream_t s, int mode = kElementFull, const LaunchTimer& timer = LaunchTimer{});
 // b^{-1} (row-major) and det b of the reference Jacobian b at every point of
 // elements [0, E): out[(8 e + q) 10 + i], i < 9 the inverse, i = 9 det b
 // ([E][8][10] doubles).  A function of X alone, formed once per mesh.
 void launch_reference_jacobians(int64_t E, const int* conn, const double* X, double* out, hipStream_t s);
 void launch_element_mixed(const BlockTable<float>& bt, int64_t E_total, const int* conn, const double* X,
-                          const double* u, float* Fip, float* sip, float* slots, int* status, hipStream_t s);
+                          const double* u, float* Fip, float* sip, float* slots, int* status, hipStream_t s,
+                          const LaunchTimer& timer = LaunchTimer{});
 
 // ---- element / material seams on gathered arrays ----
 void launch_seam_defgrad(int64_t E, const double* Xg, const double* ug, double* F, int* status, hipStream_t s);
@@ -170,7 +187,7 @@ struct StepArgs;
 // optional per-tile reaction partials; workgroup 0 first sums the fold.
 template <typename SlotT>
 void launch_node_update(const NodeArrays& na, const SlotT* slots, const StepArgs& sa, ReactionArgs rx,
-                        ReactionFold fold, hipStream_t s);
+                        ReactionFold fold, hipStream_t s, const LaunchTimer& timer = LaunchTimer{});
 // Off-nominal ensemble: sample s owns u, v, a, f at state + (4 s + j) stride,
 // j = 0..3 (exact samples first).  The closure kernel forms each approximate
 // sample's force f_s = c[s][0] f_nominal + sum_k c[s][k+1] f_exact_k

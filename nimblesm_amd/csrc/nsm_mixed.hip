@@ -74,13 +74,14 @@ __global__ __launch_bounds__(kThreads) NSM_MIXED_WAVES_ATTR void k_element_mixed
 }  // namespace
 
 void launch_element_mixed(const BlockTable<float>& bt, int64_t E_total, const int* conn, const double* X,
-                          const double* u, float* Fip, float* sip, float* slots, int* status, hipStream_t s)
+                          const double* u, float* Fip, float* sip, float* slots, int* status, hipStream_t s,
+                          const LaunchTimer& timer)
 {
   const int64_t n = bt.off[bt.count] - bt.off[0];
-  if (n <= 0) return;
+  if (n <= 0) return complete_timer(timer, s);
   const unsigned grid = (unsigned)((n + kThreads - 1) / kThreads);
-  hipLaunchKernelGGL(k_element_mixed, dim3(grid), dim3(kThreads), 0, s, bt, E_total, conn, X, u, Fip, sip, slots,
-                     status);
+  hipExtLaunchKernelGGL(k_element_mixed, dim3(grid), dim3(kThreads), 0, s, timer.start, timer.stop, 0, bt, E_total,
+                        conn, X, u, Fip, sip, slots, status);
 }
 
 }  // namespace nsm

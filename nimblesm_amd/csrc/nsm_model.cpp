@@ -288,11 +288,15 @@ int ensure_device(nsm_model* md)
   return NSM_OK;
 }
 
-int time_begin(nsm_model* md, int kernel, size_t* idx)
+// Every timing_every-th launch of each kernel is timed by an event pair.
+// With `timer` the launch records the pair itself at the kernel's start and
+// end (time_end then has nothing to do); without, time_begin and time_end
+// record it on the stream around the launch.
+int time_begin(nsm_model* md, int kernel, size_t* idx, nsm::LaunchTimer* timer = nullptr)
 {
   *idx = (size_t)-1;
+  if (timer) *timer = nsm::LaunchTimer{};
   if (!md->timing) return NSM_OK;
-  // every timing_every-th launch of each kernel is bracketed by events
   if (kernel >= 0 && kernel < 4 && (md->timing_count[kernel]++ % md->timing_every) != 0) return NSM_OK;
   if (md->event_next == md->event_pool.size()) {
     EventPair p;
@@ -300,9 +304,14 @@ int time_begin(nsm_model* md, int kernel, size_t* idx)
     NSM_HIP(hipEventCreate(&p.stop));
     md->event_pool.push_back(p);
   }
-  *idx = md->event_next++;
-  NSM_HIP(hipEventRecord(md->event_pool[*idx].start, md->stream));
-  md->timed.emplace_back(kernel, *idx);
+  const size_t i = md->event_next++;
+  md->timed.emplace_back(kernel, i);
+  if (timer) {
+    *timer = nsm::LaunchTimer{md->event_pool[i].start, md->event_pool[i].stop};
+    return NSM_OK;
+  }
+  *idx = i;
+  NSM_HIP(hipEventRecord(md->event_pool[i].start, md->stream));
   return NSM_OK;
 }
 
@@ -482,8 +491,9 @@ int launch_elements(nsm_model* md, int sample, int fp64_mode = nsm::kElementFull
   const double* u = md->node_arrays(sample).u;
   for (int b0 = 0; b0 < md->num_blocks; b0 += nsm::kMaxLaunchBlocks) {
     const int nb = std::min(nsm::kMaxLaunchBlocks, md->num_blocks - b0);
-    size_t    ti;
-    int       st = time_begin(md, NSM_KERNEL_ELEMENT, &ti);
+    size_t           ti;
+    nsm::LaunchTimer timer;
+    int              st = time_begin(md, NSM_KERNEL_ELEMENT, &ti, &timer);
     if (st) return st;
     if (md->precision == NSM_FP64) {
       nsm::BlockTable<double> bt{};
@@ -495,11 +505,11 @@ int launch_elements(nsm_model* md, int sample, int fp64_mode = nsm::kElementFull
       }
       nsm::launch_element_fp64(bt, md->E, md->conn, md->X, u, md->rjac, (double*)ip_out(md, md->Fip, sample),
                                (double*)ip_out(md, md->sip, sample), (double*)md->slots, md->status, md->stream,
-                               fp64_mode);
+                               fp64_mode, timer);
     } else {
       nsm::launch_element_mixed(float_table(md, b0, nb, sample), md->E, md->conn, md->X, u,
                                 (float*)ip_out(md, md->Fip, sample), (float*)ip_out(md, md->sip, sample),
-                                (float*)md->slots, md->status, md->stream);
+                                (float*)md->slots, md->status, md->stream, timer);
     }
     NSM_LAUNCHED();
     if ((st = time_end(md, ti))) return st;
@@ -555,17 +565,18 @@ nsm::ReactionFold reaction_fold(const nsm_model* md, int64_t fold_step)
 int launch_node(nsm_model* md, const nsm::StepArgs& sa, int64_t step, int64_t fold_step, int64_t* pending,
                 int sample)
 {
-  size_t ti;
-  int    st = time_begin(md, NSM_KERNEL_NODE, &ti);
+  size_t           ti;
+  nsm::LaunchTimer timer;
+  int              st = time_begin(md, NSM_KERNEL_NODE, &ti, &timer);
   if (st) return st;
   nsm::ReactionArgs       rx;
   const bool              want_rx = reaction_args(md, sample < 0 ? step : -1, &rx);
   const nsm::NodeArrays   na      = md->node_arrays(sample);
   const nsm::ReactionFold fold    = reaction_fold(md, fold_step);
   if (md->precision == NSM_FP64)
-    nsm::launch_node_update<double>(na, (const double*)md->slots, sa, rx, fold, md->stream);
+    nsm::launch_node_update<double>(na, (const double*)md->slots, sa, rx, fold, md->stream, timer);
   else
-    nsm::launch_node_update<float>(na, (const float*)md->slots, sa, rx, fold, md->stream);
+    nsm::launch_node_update<float>(na, (const float*)md->slots, sa, rx, fold, md->stream, timer);
   NSM_LAUNCHED();
   if ((st = time_end(md, ti))) return st;
   *pending = want_rx ? step : -1;
@@ -588,18 +599,20 @@ int launch_tiled(nsm_model* md, const nsm::StepArgs& sa, int64_t step, int64_t f
     const int nb = std::min(nsm::kMaxLaunchBlocks, md->num_blocks - b0);
     const int t0 = md->blk_tile_off[(size_t)b0], t1 = md->blk_tile_off[(size_t)(b0 + nb)];
     if (t1 <= t0) continue;  // no launch: the fold goes to the next one
-    size_t ti;
-    if ((st = time_begin(md, NSM_KERNEL_ELEMENT, &ti))) return st;
+    size_t           ti;
+    nsm::LaunchTimer timer;
+    if ((st = time_begin(md, NSM_KERNEL_ELEMENT, &ti, &timer))) return st;
     nsm::launch_element_tiled(float_table(md, b0, nb, sample), md->tile_args, t0, t1 - t0, md->X, na,
                               (float*)ip_out(md, md->Fip, sample), (float*)ip_out(md, md->sip, sample), sa, rx,
-                              fold, md->status, md->stream);
+                              fold, md->status, md->stream, timer);
     NSM_LAUNCHED();
     if ((st = time_end(md, ti))) return st;
     fold.n = 0;
   }
-  size_t ti;
-  if ((st = time_begin(md, NSM_KERNEL_NODE, &ti))) return st;
-  nsm::launch_boundary_update(md->tile_args, na, sa, rx, md->stream);
+  size_t           ti;
+  nsm::LaunchTimer timer;
+  if ((st = time_begin(md, NSM_KERNEL_NODE, &ti, &timer))) return st;
+  nsm::launch_boundary_update(md->tile_args, na, sa, rx, md->stream, timer);
   NSM_LAUNCHED();
   if ((st = time_end(md, ti))) return st;
   *pending = want_rx ? step : -1;

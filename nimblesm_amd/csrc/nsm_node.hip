@@ -484,17 +484,17 @@ void launch_prekick(const NodeArrays& na, StepTimes t, hipStream_t s)
 
 template <typename SlotT>
 void launch_node_update(const NodeArrays& na, const SlotT* slots, const StepArgs& sa, ReactionArgs rx,
-                        ReactionFold fold, hipStream_t s)
+                        ReactionFold fold, hipStream_t s, const LaunchTimer& timer)
 {
-  if (na.num_nodes <= 0) return;
-  hipLaunchKernelGGL((k_node_update<SlotT>), dim3(node_grid(na.num_nodes)), dim3(kNodeThreads), 0, s, na, slots, sa,
-                     rx, fold);
+  if (na.num_nodes <= 0) return complete_timer(timer, s);
+  hipExtLaunchKernelGGL((k_node_update<SlotT>), dim3(node_grid(na.num_nodes)), dim3(kNodeThreads), 0, s, timer.start,
+                        timer.stop, 0, na, slots, sa, rx, fold);
 }
 
 template void launch_node_update<double>(const NodeArrays&, const double*, const StepArgs&, ReactionArgs,
-                                         ReactionFold, hipStream_t);
+                                         ReactionFold, hipStream_t, const LaunchTimer&);
 template void launch_node_update<float>(const NodeArrays&, const float*, const StepArgs&, ReactionArgs, ReactionFold,
-                                        hipStream_t);
+                                        hipStream_t, const LaunchTimer&);
 
 void launch_uq_closure(const NodeArrays& na, const UqClosure& uc, const StepArgs& sa, hipStream_t s)
 {

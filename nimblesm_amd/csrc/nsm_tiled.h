@@ -83,7 +83,8 @@ unsigned      boundary_grid(int num_bnd);
 // launch keeps neither.
 void launch_element_tiled(const BlockTable<float>& bt, const TileArgs& ta, int t_begin, int t_count,
                           const double* X, const NodeArrays& na, float* Fip, float* sip, const StepArgs& sa,
-                          ReactionArgs rx, ReactionFold fold, int* status, hipStream_t s);
+                          ReactionArgs rx, ReactionFold fold, int* status, hipStream_t s,
+                          const LaunchTimer& timer = LaunchTimer{});
 // The same kernel over the tiles of every sample of an ensemble (ta: the
 // concatenated tile tables of its meshes; na: all samples' node rows, X and
 // flags of all meshes); it keeps no F or sigma.
@@ -99,6 +100,6 @@ int element_tiled_groups_per_cu(int tile_elems);
 // Step of the remaining nodes from their partials (ascending tile order).
 // Reaction partials go to rx.partials[num_tiles + workgroup].
 void launch_boundary_update(const TileArgs& ta, const NodeArrays& na, const StepArgs& sa, ReactionArgs rx,
-                            hipStream_t s);
+                            hipStream_t s, const LaunchTimer& timer = LaunchTimer{});
 
 }  // namespace nsm

@@ -613,36 +613,36 @@ unsigned boundary_grid(int num_bnd) { return (unsigned)((num_bnd + kBndThreads -
 
 template <int kThreads, typename Work>
 void launch_tiles(unsigned grid, const Work& work, const TileArgs& ta, const double* X, const NodeArrays& na,
-                  float* Fip, float* sip, ReactionArgs rx, int* status, hipStream_t s)
+                  float* Fip, float* sip, ReactionArgs rx, int* status, hipStream_t s, const LaunchTimer& t)
 {
   if (Fip)
-    hipLaunchKernelGGL((k_element_tiled<kThreads, true, Work>), dim3(grid), dim3(kThreads), 0, s, work, ta, X, na,
-                       Fip, sip, rx, status);
+    hipExtLaunchKernelGGL((k_element_tiled<kThreads, true, Work>), dim3(grid), dim3(kThreads), 0, s, t.start, t.stop,
+                          0, work, ta, X, na, Fip, sip, rx, status);
   else
-    hipLaunchKernelGGL((k_element_tiled<kThreads, false, Work>), dim3(grid), dim3(kThreads), 0, s, work, ta, X, na,
-                       Fip, sip, rx, status);
+    hipExtLaunchKernelGGL((k_element_tiled<kThreads, false, Work>), dim3(grid), dim3(kThreads), 0, s, t.start,
+                          t.stop, 0, work, ta, X, na, Fip, sip, rx, status);
 }
 
 template <typename Work>
 void launch_work(const Work& work, int count, const TileArgs& ta, const double* X, const NodeArrays& na, float* Fip,
-                 float* sip, ReactionArgs rx, int* status, hipStream_t s)
+                 float* sip, ReactionArgs rx, int* status, hipStream_t s, const LaunchTimer& t = LaunchTimer{})
 {
-  if (count <= 0) return;
+  if (count <= 0) return complete_timer(t, s);
   const unsigned grid = (unsigned)std::min(count, std::max(ta.resident_groups, 1));
   switch (ta.tile_elems) {
-    case 64: launch_tiles<64>(grid, work, ta, X, na, Fip, sip, rx, status, s); break;
-    case 128: launch_tiles<128>(grid, work, ta, X, na, Fip, sip, rx, status, s); break;
-    case 512: launch_tiles<512>(grid, work, ta, X, na, Fip, sip, rx, status, s); break;
-    default: launch_tiles<256>(grid, work, ta, X, na, Fip, sip, rx, status, s);
+    case 64: launch_tiles<64>(grid, work, ta, X, na, Fip, sip, rx, status, s, t); break;
+    case 128: launch_tiles<128>(grid, work, ta, X, na, Fip, sip, rx, status, s, t); break;
+    case 512: launch_tiles<512>(grid, work, ta, X, na, Fip, sip, rx, status, s, t); break;
+    default: launch_tiles<256>(grid, work, ta, X, na, Fip, sip, rx, status, s, t);
   }
 }
 
 void launch_element_tiled(const BlockTable<float>& bt, const TileArgs& ta, int t_begin, int t_count,
                           const double* X, const NodeArrays& na, float* Fip, float* sip, const StepArgs& sa,
-                          ReactionArgs rx, ReactionFold fold, int* status, hipStream_t s)
+                          ReactionArgs rx, ReactionFold fold, int* status, hipStream_t s, const LaunchTimer& timer)
 {
   const ModelTileWork work{bt, t_begin, t_count, sa, fold};
-  launch_work(work, t_count, ta, X, na, Fip, sip, rx, status, s);
+  launch_work(work, t_count, ta, X, na, Fip, sip, rx, status, s, timer);
 }
 
 void launch_ens_element_tiled(const EnsTileWork& work, const TileArgs& ta, const double* X, const NodeArrays& na,
@@ -671,10 +671,11 @@ int element_tiled_groups_per_cu(int tile_elems)
 }
 
 void launch_boundary_update(const TileArgs& ta, const NodeArrays& na, const StepArgs& sa, ReactionArgs rx,
-                            hipStream_t s)
+                            hipStream_t s, const LaunchTimer& timer)
 {
-  if (ta.num_bnd <= 0) return;
-  hipLaunchKernelGGL(k_boundary_update, dim3(boundary_grid(ta.num_bnd)), dim3(kBndThreads), 0, s, ta, na, sa, rx);
+  if (ta.num_bnd <= 0) return complete_timer(timer, s);
+  hipExtLaunchKernelGGL(k_boundary_update, dim3(boundary_grid(ta.num_bnd)), dim3(kBndThreads), 0, s, timer.start,
+                        timer.stop, 0, ta, na, sa, rx);
 }
 
 void launch_ens_boundary_update(const TileArgs& ta, const EnsNodes& en, const EnsRange* groups, int num_groups,
