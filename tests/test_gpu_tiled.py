@@ -325,3 +325,33 @@ def test_autotuned_tile_size(hip_device, monkeypatch, tmp_path):
     assert ep.ens.Info()["tile_elems"] == min(t, key=t.get)
     assert cache.read_text().split()[-1] == str(ep.ens.Info()["tile_elems"])
     ep.close()
+
+
+@pytest.mark.parametrize("precision,n", [("mixed", 12), ("fp64", 12), ("mixed", 4)])
+def test_kernel_timing_leaves_results_unchanged(hip_device, precision, n):
+    """Every 3rd launch of each kernel records its own start and stop events:
+    the run's bits equal an untimed run's, and 4 of the 10 steps' element and
+    node launches carry a duration.  The 4^3 cube is one tile with no
+    boundary nodes, so its boundary launch has no work and its timer still
+    completes."""
+    mats = [N.problems.STEEL, N.problems.ALUMINIUM] if n > 4 else [N.problems.STEEL]
+    prob = N.CubeProblem(n=n, materials=mats, num_steps=10).build()
+    runs = []
+    for every in (0, 3):
+        m = prob.make_model(precision)
+        if every:
+            m.EnableKernelTiming(True, every=every)
+        if n == 4:
+            assert m.AssemblyInfo()["boundary_nodes"] == 0
+        prob.make_integrator(m).Integrate()
+        runs.append([m.GetVectorNodeData(f) for f in ("displacement", "velocity", "internal_force")])
+        if every:
+            el, ne = m.KernelTime("element")
+            nd, nn = m.KernelTime("node")
+            assert ne == 4 and nn == 4, (ne, nn)
+            assert el > 0.0 and nd >= 0.0
+            if n > 4 or precision == "fp64":
+                assert nd > 0.0
+        m.close()
+    for a, b in zip(*runs):
+        assert np.array_equal(a, b)
