@@ -206,6 +206,23 @@ void launch_uq_closure(const NodeArrays& na, const UqClosure& uc, const StepArgs
 void launch_sample_history(const double* hist, int64_t n, int num_samples, double* out, hipStream_t s);
 // out = -sum_k partials[tiles[k]], k ascending (fixed tree, reproducible)
 void launch_reaction_finalize(const double* partials, const int* tiles, int num_tiles, double* out, hipStream_t s);
+// Integration-point values of mesh elements [e_begin, e_begin + count) in
+// the reference's element-data order (dst[(e - e_begin) 8 + q][c], FP64)
+// from a model's device layout:
+//   kind 0: FP64 in groups of 8 elements (fp64_ip_index);
+//   kind 1: FP32 [nc][8][E];
+//   kind 2: FP32 tile-blocked, tile t holding device elements
+//           [elem_off[t], elem_off[t + 1]); mesh element e at device
+//           position dev_elem[e].
+struct IpLayout {
+  int            kind, nc;
+  int64_t        E;
+  const int32_t* dev_elem;
+  const int32_t* elem_off;
+  int            num_tiles;
+};
+void launch_ip_gather(const void* src, const IpLayout& layout, int64_t e_begin, int64_t count, double* dst,
+                      hipStream_t s);
 // node field of width w between mesh order and a model's device order:
 // to_device: dst[map[n] w + c] = src[n w + c]; else dst[n w + c] = src[map[n] w + c]
 // (map[n] = device index of mesh node n, a permutation of [0, N))

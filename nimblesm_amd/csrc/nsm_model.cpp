@@ -180,6 +180,7 @@ struct nsm_model {
   int32_t* d_node_new   = nullptr;
   double*  d_stage      = nullptr;
   size_t   stage_values = 0;
+  int32_t* d_elem_pos   = nullptr;  // mesh element -> device element (tiled), for element-data reads
 
   // off-nominal ensemble (UQ): uq_exact exact samples, then uq_approx
   // approximate ones; sample s owns u, v, a, f at uq_state + (4 s + j) 3N
@@ -733,6 +734,21 @@ bool is_output(int64_t step, int64_t num_load_steps, int64_t output_frequency)
 }
 
 // device node array with w values per node -> host array in mesh node order
+// device staging array of at least `need` values
+int ensure_stage(nsm_model* md, size_t need)
+{
+  if (md->stage_values < need) {
+    NSM_HIP(hipStreamSynchronize(md->stream));
+    free_dev(md->d_stage);
+    md->device_bytes -= std::min(md->device_bytes, md->stage_values * sizeof(double));
+    md->stage_values = 0;
+    int st = dalloc(&md->d_stage, need, &md->device_bytes);
+    if (st) return st;
+    md->stage_values = need;
+  }
+  return NSM_OK;
+}
+
 // device map and staging array for node fields of width up to w (tiled models)
 int ensure_node_staging(nsm_model* md, size_t w)
 {
@@ -744,15 +760,45 @@ int ensure_node_staging(nsm_model* md, size_t w)
     NSM_HIP(hipMemcpyAsync(md->d_node_new, md->tiles.node_new.data(), (size_t)md->N * sizeof(int32_t),
                            hipMemcpyHostToDevice, md->stream));
   }
-  const size_t need = w * (size_t)md->N;
-  if (md->stage_values < need) {
+  return ensure_stage(md, w * (size_t)md->N);
+}
+
+// Element data (F or sigma, nc components) of every element into `out` in
+// the reference's [element][point][component] order: gathered on the device
+// into the staging array a chunk of elements at a time, then copied.
+constexpr size_t kIpStageValues = (size_t)1 << 23;  // 64 MB of FP64 per chunk
+
+int ip_field_out(nsm_model* md, const void* src, int nc, double* out)
+{
+  const int64_t E = md->E;
+  if (E == 0) return NSM_OK;
+  nsm::IpLayout L{md->precision == NSM_FP64 ? 0 : (md->tiled ? 2 : 1), nc, E, nullptr, nullptr, 0};
+  if (md->tiled) {
+    if (!md->d_elem_pos) {
+      std::vector<int32_t> pos((size_t)E);
+      for (int64_t d = 0; d < E; d++) pos[(size_t)md->tiles.perm[(size_t)d]] = (int32_t)d;
+      int32_t* p  = nullptr;
+      int      st = dalloc(&p, (size_t)E, &md->device_bytes);
+      if (st) return st;
+      md->d_elem_pos = p;
+      NSM_HIP(hipMemcpyAsync(p, pos.data(), (size_t)E * sizeof(int32_t), hipMemcpyHostToDevice, md->stream));
+      NSM_HIP(hipStreamSynchronize(md->stream));
+    }
+    L.dev_elem  = md->d_elem_pos;
+    L.elem_off  = md->tile_args.elem_off;
+    L.num_tiles = md->tiles.num_tiles;
+  }
+  const int64_t per   = 8 * (int64_t)nc;
+  const int64_t chunk = std::min<int64_t>(E, std::max<int64_t>(1, (int64_t)kIpStageValues / per));
+  int           st    = ensure_stage(md, (size_t)(chunk * per));
+  if (st) return st;
+  for (int64_t e0 = 0; e0 < E; e0 += chunk) {
+    const int64_t n = std::min(chunk, E - e0);
+    nsm::launch_ip_gather(src, L, e0, n, md->d_stage, md->stream);
+    NSM_LAUNCHED();
+    NSM_HIP(hipMemcpyAsync(out + e0 * per, md->d_stage, (size_t)(n * per) * sizeof(double), hipMemcpyDeviceToHost,
+                           md->stream));
     NSM_HIP(hipStreamSynchronize(md->stream));
-    free_dev(md->d_stage);
-    md->device_bytes -= std::min(md->device_bytes, md->stage_values * sizeof(double));
-    md->stage_values = 0;
-    int st = dalloc(&md->d_stage, need, &md->device_bytes);
-    if (st) return st;
-    md->stage_values = need;
   }
   return NSM_OK;
 }
@@ -1173,6 +1219,7 @@ int nsm_model_destroy(nsm_model* md)
   free_dev(md->uq_bc_table);
   free_dev(md->d_node_new);
   free_dev(md->d_stage);
+  free_dev(md->d_elem_pos);
   for (void* p : md->tile_bufs) (void)hipFree(p);
   for (auto& p : md->event_pool) {
     (void)hipEventDestroy(p.start);
@@ -1570,26 +1617,9 @@ int nsm_model_get_field(nsm_model* md, int field, double* out)
     case NSM_FIELD_DEFORMATION_GRADIENT:
     case NSM_FIELD_STRESS: {
       // device layout (for_each_ip) -> host [E][ipt][comp] (the reference's element-data order)
-      const int    nc  = field == NSM_FIELD_DEFORMATION_GRADIENT ? 9 : 6;
-      const void*  p   = field == NSM_FIELD_DEFORMATION_GRADIENT ? md->Fip : md->sip;
-      const size_t E   = (size_t)md->E;
-      if (E == 0) return NSM_OK;
-      // device values: FP64 groups of 8 elements cover E rounded up to 8
-      cnt = (size_t)nc * 8 * (md->precision == NSM_FP64 ? (size_t)nsm::round_up8(md->E) : E);
-      std::vector<double> tmp(cnt);
-      if (md->precision == NSM_FP64) {
-        NSM_HIP(hipMemcpyAsync(tmp.data(), p, cnt * sizeof(double), hipMemcpyDeviceToHost, md->stream));
-        NSM_HIP(hipStreamSynchronize(md->stream));
-      } else {
-        std::vector<float> tf(cnt);
-        NSM_HIP(hipMemcpyAsync(tf.data(), p, cnt * sizeof(float), hipMemcpyDeviceToHost, md->stream));
-        NSM_HIP(hipStreamSynchronize(md->stream));
-        for (size_t i = 0; i < cnt; i++) tmp[i] = tf[i];
-      }
-      for_each_ip(md, nc, [&](int64_t i, int64_t e, int c, int q) {
-        out[((size_t)element_of(md, e) * 8 + (size_t)q) * (size_t)nc + (size_t)c] = tmp[(size_t)i];
-      });
-      return NSM_OK;
+      const int   nc = field == NSM_FIELD_DEFORMATION_GRADIENT ? 9 : 6;
+      const void* p  = field == NSM_FIELD_DEFORMATION_GRADIENT ? md->Fip : md->sip;
+      return ip_field_out(md, p, nc, out);
     }
     default: return fail(NSM_ERR_ARG, "unknown field %d", field);
   }

@@ -189,6 +189,36 @@ __global__ __launch_bounds__(kNodeThreads) void k_permute_nodes(const double* __
     for (int c = 0; c < w; c++) dst[n * w + c] = src[d * w + c];
 }
 
+// One value per thread, written in output order.
+__global__ __launch_bounds__(kNodeThreads) void k_ip_gather(const void* __restrict__ src, IpLayout L,
+                                                            int64_t e_begin, int64_t count, double* __restrict__ dst)
+{
+  const int64_t v   = (int64_t)blockIdx.x * kNodeThreads + threadIdx.x;
+  const int     per = 8 * L.nc;
+  if (v >= count * per) return;
+  const int64_t e = e_begin + v / per;
+  const int     r = (int)(v % per), q = r / L.nc, c = r % L.nc;
+  double        x;
+  if (L.kind == 0) {
+    x = static_cast<const double*>(src)[fp64_ip_index(e, L.nc, c, q)];
+  } else if (L.kind == 1) {
+    x = static_cast<const float*>(src)[(int64_t)(c * 8 + q) * L.E + e];
+  } else {
+    const int64_t d = L.dev_elem[e];
+    int           lo = 0, hi = L.num_tiles;  // elem_off[lo] <= d < elem_off[hi]
+    while (hi - lo > 1) {
+      const int mid = (lo + hi) >> 1;
+      if (L.elem_off[mid] <= d)
+        lo = mid;
+      else
+        hi = mid;
+    }
+    const int64_t e0 = L.elem_off[lo], n = L.elem_off[lo + 1] - e0;
+    x = static_cast<const float*>(src)[8 * L.nc * e0 + (c * 4 + q / 2) * 2 * n + 2 * (d - e0) + (q & 1)];
+  }
+  dst[v] = x;
+}
+
 // UQ closure (UqModel::ApplyClosure, src/uq/nimble_uq.cc:327-346) fused with
 // the approximate samples' node update: per node, f_s = f_nominal * c0, then
 // f_s += f_exact_k * c_{k+1} for k ascending, then node_step with the
@@ -425,6 +455,15 @@ void launch_ens_node_fp64(const EnsNodes& en, const EnsRange* chunks, int num_ch
   if (num_chunks <= 0) return;
   hipLaunchKernelGGL(k_ens_node_fp64, dim3((unsigned)num_chunks), dim3(kNodeThreads), 0, s, en, chunks, node_ptr,
                      node_ent, slots, es, rx, fold);
+}
+
+void launch_ip_gather(const void* src, const IpLayout& layout, int64_t e_begin, int64_t count, double* dst,
+                      hipStream_t s)
+{
+  const int64_t values = count * 8 * layout.nc;
+  if (values <= 0) return;
+  hipLaunchKernelGGL(k_ip_gather, dim3((unsigned)((values + kNodeThreads - 1) / kNodeThreads)), dim3(kNodeThreads), 0,
+                     s, src, layout, e_begin, count, dst);
 }
 
 void launch_permute_nodes(const double* src, const int32_t* map, int64_t N, int w, bool to_device, double* dst,
