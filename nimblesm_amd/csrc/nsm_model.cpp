@@ -177,10 +177,10 @@ struct nsm_model {
   std::vector<void*>   tile_bufs;
   // host transfers of node fields: mesh -> device node map and a staging
   // array in mesh order (allocated at the first transfer)
-  int32_t* d_node_new   = nullptr;
-  double*  d_stage      = nullptr;
-  size_t   stage_values = 0;
-  int32_t* d_elem_pos   = nullptr;  // mesh element -> device element (tiled), for element-data reads
+  const int32_t* d_node_new   = nullptr;  // in tile_bufs
+  double*        d_stage      = nullptr;
+  size_t         stage_values = 0;
+  const int32_t* d_elem_pos   = nullptr;  // mesh element -> device element (tiled; in tile_bufs)
 
   // off-nominal ensemble (UQ): uq_exact exact samples, then uq_approx
   // approximate ones; sample s owns u, v, a, f at uq_state + (4 s + j) 3N
@@ -753,12 +753,8 @@ int ensure_stage(nsm_model* md, size_t need)
 int ensure_node_staging(nsm_model* md, size_t w)
 {
   if (!md->d_node_new) {
-    int32_t* p = nullptr;
-    int      st = dalloc(&p, (size_t)md->N, &md->device_bytes);
+    int st = upload(md, md->tiles.node_new, &md->d_node_new);
     if (st) return st;
-    md->d_node_new = p;
-    NSM_HIP(hipMemcpyAsync(md->d_node_new, md->tiles.node_new.data(), (size_t)md->N * sizeof(int32_t),
-                           hipMemcpyHostToDevice, md->stream));
   }
   return ensure_stage(md, w * (size_t)md->N);
 }
@@ -777,12 +773,8 @@ int ip_field_out(nsm_model* md, const void* src, int nc, double* out)
     if (!md->d_elem_pos) {
       std::vector<int32_t> pos((size_t)E);
       for (int64_t d = 0; d < E; d++) pos[(size_t)md->tiles.perm[(size_t)d]] = (int32_t)d;
-      int32_t* p  = nullptr;
-      int      st = dalloc(&p, (size_t)E, &md->device_bytes);
+      int st = upload(md, pos, &md->d_elem_pos);
       if (st) return st;
-      md->d_elem_pos = p;
-      NSM_HIP(hipMemcpyAsync(p, pos.data(), (size_t)E * sizeof(int32_t), hipMemcpyHostToDevice, md->stream));
-      NSM_HIP(hipStreamSynchronize(md->stream));
     }
     L.dev_elem  = md->d_elem_pos;
     L.elem_off  = md->tile_args.elem_off;
@@ -1217,9 +1209,7 @@ int nsm_model_destroy(nsm_model* md)
   free_dev(md->uq_state);
   free_dev(md->uq_coeff);
   free_dev(md->uq_bc_table);
-  free_dev(md->d_node_new);
   free_dev(md->d_stage);
-  free_dev(md->d_elem_pos);
   for (void* p : md->tile_bufs) (void)hipFree(p);
   for (auto& p : md->event_pool) {
     (void)hipEventDestroy(p.start);
