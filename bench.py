@@ -185,6 +185,34 @@ def cpu_baseline(prob, steps, runs=5):
     t0 = time.perf_counter()
     serial.advance(1)
     ts = time.perf_counter() - t0
+    # NimbleSM's own element and material code (oracle/_ref, compiled from the
+    # reference's sources where they were present), serial as the reference's
+    # CPU path runs: F, stress and forces of a sample of the workload's
+    # elements, gathered from the current state
+    reference = None
+    R = O.ref_lib()
+    if R is not None:
+        blk = prob.blocks()[0]
+        S = min(262144, len(blk.conn))
+        conn = np.asarray(blk.conn[:S], np.int64)
+        Xg = np.ascontiguousarray(prob.X[conn])
+        ug = np.ascontiguousarray(run.u[conn])
+        F, sig, f = np.zeros((S, 8, 9)), np.zeros((S, 8, 6)), np.zeros((S, 8, 3))
+        K, G = blk.material.bulk_modulus, blk.material.shear_modulus
+        rt = []
+        for _ in range(3):
+            t0 = time.perf_counter()
+            R.ref_deformation_gradients(S, Xg, ug, F)
+            R.ref_stress(8 * S, K, G, F, sig)
+            R.ref_nodal_forces(S, Xg, ug, sig, f)
+            rt.append(time.perf_counter() - t0)
+        reference = {
+            "value": S / statistics.median(rt), "unit": "element-evaluations/s", "cores": 1, "kind": "reference",
+            "sample": f"F, Neo-Hookean stress and nodal forces of NimbleSM's own element/material code on {S} "
+                      f"pre-gathered elements of {prob.name}'s first block, serial, median of 3 runs "
+                      f"({sum(rt):.1f} s in all); no gather, scatter or node update",
+            "runs_s": rt,
+        }
     return {
         "value": prob.num_elem * per / med,
         "unit": "element-steps/s",
@@ -198,6 +226,7 @@ def cpu_baseline(prob, steps, runs=5):
         "cpu_model": _cpu_model(),
         "nproc": os.cpu_count(),
         "omp_num_threads": os.environ.get("OMP_NUM_THREADS"),
+        "reference_code": reference,
     }
 
 
