@@ -273,10 +273,12 @@ int nsm_model_assembly_info(nsm_model* model, int* tiled, int64_t* num_tiles, in
 int nsm_model_tile_elems(nsm_model* model, int* tile_elems);
 
 int nsm_model_synchronize(nsm_model* model);
-/* Launch-duration statistics gathered by HIP events around launches of
- * `kernel` while timing is enabled: average milliseconds and launch count.
- * enable = n > 0 brackets every n-th launch of each kernel (1: every launch;
- * each event pair adds a few microseconds to the stream), 0 disables. */
+/* Launch-duration statistics from HIP events of launches of `kernel` while
+ * timing is enabled: average milliseconds and launch count.  enable = n > 0
+ * times every n-th launch of each kernel (1: every launch), 0 disables.  The
+ * element and node launches record their events themselves at the kernel's
+ * start and end; the reaction and closure launches are bracketed by events
+ * on the stream, which add a few microseconds each. */
 int nsm_model_enable_kernel_timing(nsm_model* model, int enable);
 int nsm_model_kernel_time(nsm_model* model, int kernel, double* avg_ms, int64_t* launches);
 /* Sizes of the model: nodes, elements, CSR entries, bytes of device memory. */

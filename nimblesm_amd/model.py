@@ -326,7 +326,7 @@ class ModelData:
         check(lib().nsm_model_synchronize(self._h))
 
     def EnableKernelTiming(self, enable=True, every: int = 1):
-        """HIP events around every `every`-th launch of each kernel."""
+        """HIP events on every `every`-th launch of each kernel."""
         check(lib().nsm_model_enable_kernel_timing(self._h, int(every) if enable else 0))
 
     def KernelTime(self, kernel: str):
