@@ -2,11 +2,15 @@
 //
 // Device layouts:
 //   node vectors          [N][3] double (the reference's AoS node layout)
-//   integration points    element-fastest "LayoutLeft": F [9][8][E],
-//                         sigma [6][8][E] (component, Gauss point, element)
-//   element force slots   [3][8][E] (component, local node, element)
-//   node -> element CSR   node_ent[p] = k*E + e for the p-th incidence
-//                         (element e, local node k), ascending e per node
+//   integration points    FP64: groups of 8 elements (fp64_ip_index); FP32:
+//                         [c][8][E] (component, Gauss point, element), or
+//                         tile-blocked in tiled models (launch_element_tiled)
+//   element force slots   [3][slots] (component, slot): slot fp64_slot(e, k)
+//                         in FP64 models (slots = 8 E, E rounded up to 8),
+//                         k*E + e in the untiled mixed path; tiled models
+//                         keep none
+//   node -> element CSR   node_ent[p] = slot of the p-th incidence (element
+//                         e, local node k), ascending e per node
 #pragma once
 
 #include <hip/hip_ext.h>
